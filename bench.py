@@ -1,0 +1,148 @@
+"""Benchmark: 2D DIM helium shock-bubble cell-updates/s on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload helium_fp64|helium_fp32] [--gpus 1]
+
+A "step" is one dim::advance_one_step (CFL dt, x sweep, ring BCs, y sweep,
+ring BCs) over the whole grid, state resident in HBM.  Prints one JSON line.
+
+value = cells x steps / wall seconds of the timed region (the reference's
+metric, tools/quant/report_builders.hpp:298-300).  roofline: the sweep
+kernel's algorithmic bytes per launch (each directional sweep reads and
+writes the nv conserved fields once: 2 * nv * bytes/value per cell) over its
+mean device duration from HIP events on the library's stream.  cpu_baseline:
+the reference's own OpenMP solver (oracle/_ref/libdimref.so, built from the
+reference's sources) timed on this host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (grid, precision, value bytes)
+    "helium_fp64": ((2048, 1024), "fp64", 8),
+    "helium_fp32": ((4096, 2048), "fp32_mixed", 4),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="helium_fp64", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def traffic_for(workload):
+    """HBM bytes per sweep launch from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_sweep_launch")
+
+
+def cpu_baseline(case, nsteps):
+    """The reference's OpenMP solver on this host's cores (bounded sample)."""
+    from oracle.pyoracle import Checker, make_params, ref_available
+
+    if not ref_available():
+        return None
+    threads = min(16, os.cpu_count() or 1)
+    ref = Checker("ref")
+    p = make_params(case.dim, case.materials)
+    _, _, secs = ref.advance(p, case.N, case.dx, case.bc_lo, case.bc_hi, case.U, case.cfl, 1.0,
+                             nsteps=nsteps, threads=threads)
+    return {
+        "value": case.ncells * nsteps / secs,
+        "unit": "cell-updates/s",
+        "cores": threads,
+        "kind": "reference",
+        "sample": f"{nsteps} FP64 steps of helium {case.N[0]}x{case.N[1]} from the initial state, "
+                  f"reference dim::advance_one_step, OpenMP guided,1, {secs:.1f} s",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "bench.py measures one GPU; multi-GPU runs are not part of this build",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    from diffuse_vs_gfm_mms_amd import dimhip, workloads
+
+    grid, precision, vbytes = WORKLOADS[args.workload]
+    case = workloads.helium_shock_bubble_2d(*grid)
+    t_end = 1.0  # far beyond the steps taken: dt is CFL-limited throughout
+
+    solver = dimhip.Solver.from_case(case, precision=precision)
+    t, n = solver.advance(args.warmup, case.cfl, t_end, 0.0, use_graph=False)
+    solver.synchronize()
+    solver.set_timing(True)
+    t0 = time.perf_counter()
+    t, n = solver.advance(args.steps, case.cfl, t_end, t, use_graph=False)
+    solver.synchronize()
+    wall = time.perf_counter() - t0
+    ms, counts = solver.kernel_times()
+    solver.set_timing(False)
+    if n != args.steps:
+        raise RuntimeError(f"took {n} steps, expected {args.steps}")
+    solver.close()
+
+    cells = case.ncells
+    value = cells * args.steps / wall
+    sweep_ms = ms[0]
+    bytes_per_launch = 2 * case.nv * vbytes * cells
+    achieved = bytes_per_launch / (sweep_ms * 1e-3) / 1e9
+    out = {
+        "metric": "cell-updates/s, 2D DIM MUSCL-Hancock+HLLC, FP32-mixed vs FP64; % HBM peak",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"helium shock-bubble {grid[0]}x{grid[1]}, {precision}, split x->y, CFL 0.8",
+            "grid": list(grid),
+            "precision": precision,
+            "cells": cells,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_for(args.workload),
+            "kernel": "sweep (x and y launches averaged)",
+            "mean_ms": {"sweep": ms[0], "boundary": ms[1], "cfl": ms[2]},
+            "launches": {"sweep": counts[0], "boundary": counts[1], "cfl": counts[2]},
+            "bytes_per_launch": bytes_per_launch,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(workloads.helium_shock_bubble_2d(2048, 1024), args.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,692 @@
+// dim_fp64.hpp — FP64 device arithmetic of the DIM update, faithful to the
+// reference's evaluation order so results match it bit for bit.
+//
+// Compiled with -ffp-contract=off: every + - * / sqrt below is one IEEE-754
+// double operation rounded to nearest, as on the reference's x86-64 SSE2
+// build.  std::max/std::min are restated as (a<b)?b:a and (b<a)?b:a because
+// their NaN behaviour differs from fmax/fmin.
+//
+// Templates: D = spatial dimension (1..3), M = number of materials (1..4).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dim_params.hpp"
+
+namespace dimhip {
+
+// ---- src/math/numerical_safety.hpp -------------------------------------
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ bool finite(double x) { return __builtin_isfinite(x); }
+// numerical_safety.hpp:12-18
+__device__ __forceinline__ double clamp_min(double x, double tol = 1e-10)
+{
+    return finite(x) ? smax(x, tol) : tol;
+}
+// numerical_safety.hpp:21-24
+__device__ __forceinline__ double finite_or(double x, double fb = 0.0) { return finite(x) ? x : fb; }
+// numerical_safety.hpp:27-39
+__device__ __forceinline__ double safe_denom(double d, double tol = 1e-10)
+{
+    if (!finite(d)) return (d < 0.0) ? -tol : tol;
+    if (__builtin_fabs(d) < tol) return (d < 0.0) ? -tol : tol;
+    return d;
+}
+// numerical_safety.hpp:41-45
+__device__ __forceinline__ double safe_div(double n, double d, double tol = 1e-10)
+{
+    return finite_or(n) / safe_denom(d, tol);
+}
+
+template<int D, int M>
+struct State {
+    double m[M];
+    double mom[D];
+    double E;
+    double a[M > 1 ? M - 1 : 1];
+};
+
+template<int D, int M>
+struct Prim {
+    double rho[M];
+    double alpha[M];
+    double vel[D];
+    double p;
+};
+
+template<int D, int M>
+struct Flux {
+    double m[M];
+    double mom[D];
+    double E;
+};
+
+// ---- per-material EoS (src/euler/eos.hpp) --------------------------------
+// pressure<1> of a zero-momentum Conserved<1> (eos.hpp:451-466, 555-571)
+__device__ __forceinline__ double mat_pressure_c1(const EosParams& P, int k, double rho_in, double E)
+{
+    const double rho = clamp_min(rho_in);
+    const double kinetic = 0.5 * safe_div(0.0, rho);
+    if (P.kind[k] == EOS_STIFFENED)
+        return clamp_min(P.gm1[k] * (E - kinetic) - P.gpinf[k]);
+    return clamp_min(P.gm1[k] * (E - kinetic));
+}
+
+// internal_energy (eos.hpp:482-489, 587-595); gm1 = gamma-1, gpinf = gamma*p_inf
+__device__ __forceinline__ double mat_internal_energy(const EosParams& P, int k, double rho, double p)
+{
+    if (P.kind[k] == EOS_STIFFENED) return safe_div(p + P.gpinf[k], P.gm1[k] * rho);
+    return safe_div(p, P.gm1[k] * rho);
+}
+
+// eos.hpp:16-27
+__device__ __forceinline__ double material_pressure_from_density_energy(const EosParams& P, int k,
+                                                                         double rho, double e)
+{
+    const double r = clamp_min(rho);
+    return mat_pressure_c1(P, k, r, r * smax(e, 0.0));
+}
+
+// eos.hpp:29-40 with sound_speed<1> (eos.hpp:469-479, 574-583)
+__device__ __forceinline__ double material_sound_speed(const EosParams& P, int k, double rho, double p)
+{
+    const double r = clamp_min(rho);
+    const double E = r * mat_internal_energy(P, k, r, p);
+    const double r2 = clamp_min(r);
+    const double pp = mat_pressure_c1(P, k, r, E);
+    if (P.kind[k] == EOS_STIFFENED) return __builtin_sqrt(P.gamma[k] * safe_div(pp + P.pinf[k], r2));
+    return __builtin_sqrt(P.gamma[k] * safe_div(pp, r2));
+}
+
+// eos.hpp:42-57
+__device__ __forceinline__ double material_energy_pressure_derivative(const EosParams& P, int k,
+                                                                      double rho, double p)
+{
+    rho = clamp_min(rho);
+    p = clamp_min(p);
+    const double dp = smax(1e-6 * smax(__builtin_fabs(p), 1.0), 1e-8);
+    const double p_lo = smax(p - dp, 1e-12);
+    const double p_hi = p + dp;
+    const double e_lo = mat_internal_energy(P, k, rho, p_lo);
+    const double e_hi = mat_internal_energy(P, k, rho, p_hi);
+    return smax(safe_div(e_hi - e_lo, p_hi - p_lo, 1e-14), 1e-14);
+}
+
+// ---- mixture EoS (src/dim/eos.hpp) ---------------------------------------
+// Loop-invariant part of the residual of the pressure bisection:
+// coef_k = max(finite_or(alpha_k),0) * clamp_min(rho_k), den_k = (gamma_k-1)*clamp_min(rho_k)
+template<int M>
+struct MixCoef {
+    double coef[M];
+    double den[M];
+};
+
+template<int M>
+__device__ __forceinline__ MixCoef<M> mix_coef(const EosParams& P, const double* alpha, const double* rho)
+{
+    MixCoef<M> c;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const double rk = clamp_min(rho[k]);
+        c.coef[k] = smax(finite_or(alpha[k]), 0.0) * rk;
+        c.den[k] = safe_denom(P.gm1[k] * rk);
+    }
+    return c;
+}
+
+// eos.hpp:59-75 (mixture_internal_energy_density) with the invariants hoisted
+template<int M>
+__device__ __forceinline__ double mix_energy(const EosParams& P, const MixCoef<M>& c, double p)
+{
+    const double pk = clamp_min(p);
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const double num = (P.kind[k] == EOS_STIFFENED) ? (pk + P.gpinf[k]) : pk;
+        s += c.coef[k] * (finite_or(num) / c.den[k]);
+    }
+    return finite_or(s);
+}
+
+// eos.hpp:77-143 — bracket doubling + bisection.  Every evaluated midpoint
+// and comparison is the reference's; only loop-invariant subexpressions are
+// hoisted (mix_coef), which does not change any rounded value.
+template<int M>
+__device__ double pressure_from_internal_energy(const EosParams& P, double rho_total, double e,
+                                                const double* alpha, const double* rho)
+{
+    rho_total = clamp_min(rho_total);
+    e = finite_or(e);
+    const double target = finite_or(rho_total * e);
+    double fallback = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const double ek = target / rho_total;
+        fallback += smax(finite_or(alpha[k]), 0.0) *
+                    material_pressure_from_density_energy(P, k, clamp_min(rho[k]), ek);
+    }
+    fallback = clamp_min(fallback);
+
+    const MixCoef<M> c = mix_coef<M>(P, alpha, rho);
+    double lo = 1e-12;
+    double hi = smax(1.0, fallback);
+    const double flo = mix_energy<M>(P, c, lo) - target;
+    double fhi = mix_energy<M>(P, c, hi) - target;
+    for (int n = 0; n < 80 && finite(fhi) && fhi < 0.0; ++n) {
+        hi *= 2.0;
+        fhi = mix_energy<M>(P, c, hi) - target;
+    }
+    if (!finite(flo) || !finite(fhi) || !(flo <= 0.0 && fhi >= 0.0)) return clamp_min(fallback);
+
+    const double tol = 1e-10 * smax(1.0, target);
+    for (int n = 0; n < 80; ++n) {
+        const double mid = 0.5 * (lo + hi);
+        const double fmid = mix_energy<M>(P, c, mid) - target;
+        if (!finite(fmid)) return clamp_min(fallback);
+        if (__builtin_fabs(fmid) <= tol) return clamp_min(mid);
+        if (fmid > 0.0) hi = mid;
+        else lo = mid;
+    }
+    return clamp_min(0.5 * (lo + hi));
+}
+
+// eos.hpp:163-181 via eos.hpp:59-75
+template<int M>
+__device__ __forceinline__ double internal_energy_from_pressure(const EosParams& P, double rho_total,
+                                                                double p, const double* alpha,
+                                                                const double* rho)
+{
+    rho_total = clamp_min(rho_total);
+    p = clamp_min(p);
+    const MixCoef<M> c = mix_coef<M>(P, alpha, rho);
+    return mix_energy<M>(P, c, p) / rho_total;
+}
+
+// eos.hpp:195-221 (Wood) and eos.hpp:223-275 (allaire / equal_velocity)
+template<int M>
+__device__ double mixture_sound_speed(const EosParams& P, double rho_total, double p,
+                                      const double* alpha, const double* rho)
+{
+    if (P.lambda_model == LAMBDA_KAPILA) {
+        rho_total = clamp_min(rho_total);
+        p = clamp_min(p);
+        double beta_mix = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const double rho_k = clamp_min(rho[k]);
+            const double ck = material_sound_speed(P, k, rho_k, p);
+            beta_mix += smax(finite_or(alpha[k]), 0.0) / safe_denom(rho_k * ck * ck, 1e-14);
+        }
+        return __builtin_sqrt(1.0 / (rho_total * safe_denom(beta_mix, 1e-14)));
+    }
+    double cs[M], xi[M];
+    double xi_mix = 0.0, inverse_sound_sum = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const double alpha_k = smax(finite_or(alpha[k]), 0.0);
+        const double rho_k = clamp_min(rho[k]);
+        cs[k] = material_sound_speed(P, k, rho_k, p);
+        xi[k] = rho_k * material_energy_pressure_derivative(P, k, rho_k, p);
+        xi_mix += alpha_k * xi[k];
+        inverse_sound_sum += alpha_k / safe_denom(cs[k], 1e-14);
+    }
+    xi_mix = safe_denom(xi_mix, 1e-14);
+    inverse_sound_sum = safe_denom(inverse_sound_sum, 1e-14);
+    double rc2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const double alpha_k = smax(finite_or(alpha[k]), 0.0);
+        double lambda_k = 1.0;
+        if (P.lambda_model == LAMBDA_EQUAL_VELOCITY)
+            lambda_k = safe_div(1.0 / safe_denom(cs[k], 1e-14), inverse_sound_sum, 1e-14);
+        rc2 += lambda_k * alpha_k * xi[k] * cs[k] * cs[k] / xi_mix;
+    }
+    return __builtin_sqrt(smax(rc2 / clamp_min(rho_total), 1e-14));
+}
+
+// ---- state algebra (src/dim/primitives.hpp) ---------------------------
+// primitives.hpp:18-42
+template<int D, int M>
+__device__ __forceinline__ void full_alpha(const State<D, M>& U, double* alpha)
+{
+    double sum = 0.0;
+#pragma unroll
+    for (int k = 0; k < M - 1; ++k) {
+        alpha[k] = U.a[k];
+        sum += alpha[k];
+    }
+    alpha[M - 1] = 1.0 - sum;
+}
+
+// primitives.hpp:45-70
+template<int M>
+__device__ __forceinline__ void sanitise_alpha(double* alpha, double floor)
+{
+    double sum = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        alpha[k] = finite_or(alpha[k], floor);
+        alpha[k] = smax(alpha[k], floor);
+        sum += alpha[k];
+    }
+    if (!finite(sum) || sum <= 0.0) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) alpha[k] = (k == 0) ? 1.0 : 0.0;
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) alpha[k] /= sum;
+}
+
+// primitives.hpp:99-108
+template<int D, int M>
+__device__ __forceinline__ double total_density(const State<D, M>& U)
+{
+    double r = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) r += finite_or(U.m[k]);
+    return r;
+}
+
+// primitives.hpp:111-162
+template<int D, int M>
+__device__ __forceinline__ void cons_to_prim(const EosParams& P, const State<D, M>& U, Prim<D, M>& R)
+{
+    full_alpha<D, M>(U, R.alpha);
+    sanitise_alpha<M>(R.alpha, 1e-12);
+    double rho_total = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) rho_total += finite_or(U.m[k]);
+    rho_total = smax(rho_total, 1e-12);
+    double v2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        R.vel[d] = safe_div(U.mom[d], rho_total);
+        v2 += R.vel[d] * R.vel[d];
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+        R.rho[k] = smax(safe_div(finite_or(U.m[k]), smax(R.alpha[k], 1e-12)), 1e-12);
+    const double kinetic = 0.5 * rho_total * v2;
+    const double ie = safe_div(finite_or(U.E, kinetic) - kinetic, rho_total);
+    R.p = smax(pressure_from_internal_energy<M>(P, rho_total, ie, R.alpha, R.rho), 1e-12);
+}
+
+// primitives.hpp:166-212
+template<int D, int M>
+__device__ __forceinline__ void prim_to_cons(const EosParams& P, const Prim<D, M>& Pin, State<D, M>& U)
+{
+    double alpha[M], rho[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) alpha[k] = Pin.alpha[k];
+    sanitise_alpha<M>(alpha, 1e-12);
+    double rho_total = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        rho[k] = smax(Pin.rho[k], 1e-12);
+        U.m[k] = alpha[k] * rho[k];
+        rho_total += U.m[k];
+    }
+    rho_total = smax(rho_total, 1e-12);
+    const double p = smax(Pin.p, 1e-12);
+    double v2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        U.mom[d] = rho_total * Pin.vel[d];
+        v2 += Pin.vel[d] * Pin.vel[d];
+    }
+    const double ie = internal_energy_from_pressure<M>(P, rho_total, p, alpha, rho);
+    U.E = rho_total * ie + 0.5 * rho_total * v2;
+#pragma unroll
+    for (int k = 0; k < M - 1; ++k) U.a[k] = alpha[k];
+}
+
+// primitives.hpp:216-322
+template<int D, int M>
+__device__ __forceinline__ void alpha_rhs_coefficients(const EosParams& P, const Prim<D, M>& Pr,
+                                                       double* rhs)
+{
+    const double floor = 1e-12;
+#pragma unroll
+    for (int k = 0; k < M; ++k) rhs[k] = Pr.alpha[k];
+    sanitise_alpha<M>(rhs, 0.0);
+    if (P.lambda_model == LAMBDA_ALLAIRE) return;
+    const double pp = smax(finite_or(Pr.p), floor);
+    if (P.lambda_model == LAMBDA_EQUAL_VELOCITY) {
+        double inv_sum = 0.0;
+        double cs[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            cs[k] = floor;
+            const double alpha_k = smax(finite_or(rhs[k]), 0.0);
+            if (alpha_k <= floor || alpha_k < smax(P.alpha_source_floor, 0.0)) continue;
+            cs[k] = material_sound_speed(P, k, smax(finite_or(Pr.rho[k]), floor), pp);
+            inv_sum += alpha_k / safe_denom(cs[k], 1e-14);
+        }
+        if (inv_sum <= floor || !finite(inv_sum)) return;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const double alpha_k = smax(finite_or(rhs[k]), 0.0);
+            if (alpha_k <= floor) {
+                rhs[k] = 0.0;
+                continue;
+            }
+            rhs[k] = alpha_k * safe_div(1.0 / safe_denom(cs[k], 1e-14), inv_sum, 1e-14);
+        }
+        return;
+    }
+    const double source_floor = smax(P.alpha_source_floor, 0.0);
+    double beta_mix = 0.0;
+    double imp[M];
+    bool active[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        imp[k] = floor;
+        active[k] = false;
+        const double alpha_k = smax(finite_or(rhs[k]), 0.0);
+        if (alpha_k <= floor || alpha_k < source_floor) continue;
+        active[k] = true;
+        const double rho_k = smax(finite_or(Pr.rho[k]), floor);
+        const double c_k = material_sound_speed(P, k, rho_k, pp);
+        imp[k] = smax(rho_k * c_k * c_k, floor);
+        beta_mix += alpha_k / imp[k];
+    }
+    if (beta_mix <= floor || !finite(beta_mix)) return;
+    const double q_mix = 1.0 / beta_mix;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const double alpha_k = smax(finite_or(rhs[k]), 0.0);
+        if (!active[k] || alpha_k <= floor) {
+            rhs[k] = 0.0;
+            continue;
+        }
+        rhs[k] = alpha_k + alpha_k * (q_mix / imp[k] - 1.0);
+    }
+}
+
+// primitives.hpp:326-457.  The reference runs a pressure bisection
+// (pressure_before) on every call but reads it only when inactive mass is
+// removed; here it is evaluated on that branch only, from the same inputs.
+template<int D, int M>
+__device__ __forceinline__ void repair_state(const EosParams& P, State<D, M>& U)
+{
+    const double mass_floor = 1e-12, p_floor = 1e-12, inactive = 1e-10;
+#pragma unroll
+    for (int k = 0; k < M; ++k) U.m[k] = smax(finite_or(U.m[k]), 0.0);
+#pragma unroll
+    for (int k = 0; k < M - 1; ++k) U.a[k] = finite_or(U.a[k]);
+#pragma unroll
+    for (int d = 0; d < D; ++d) U.mom[d] = finite_or(U.mom[d]);
+    U.E = finite_or(U.E);
+
+    const double rho_before = smax(total_density<D, M>(U), mass_floor);
+    double vel_before[D];
+    double v2_before = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        vel_before[d] = safe_div(U.mom[d], rho_before);
+        v2_before += vel_before[d] * vel_before[d];
+    }
+    // alpha_before equals alpha_full below: both are sanitise(full_alpha(U.alpha), 0)
+    // of the same (already finite_or'd) values
+    double alpha_full[M];
+    full_alpha<D, M>(U, alpha_full);
+    sanitise_alpha<M>(alpha_full, 0.0);
+#pragma unroll
+    for (int k = 0; k < M - 1; ++k) U.a[k] = alpha_full[k];
+
+    bool removed = false;
+    double m_before[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        m_before[k] = U.m[k];
+        if (alpha_full[k] <= inactive && U.m[k] > mass_floor) {
+            U.m[k] = 0.0;
+            removed = true;
+        }
+    }
+    double rho_total = total_density<D, M>(U);
+    if (!finite(rho_total) || rho_total <= 0.0) {
+        int dom = 0;
+#pragma unroll
+        for (int k = 1; k < M; ++k)
+            if (alpha_full[k] > alpha_full[dom]) dom = k;
+#pragma unroll
+        for (int k = 0; k < M; ++k)
+            if (k == dom) U.m[k] = mass_floor;
+        rho_total = mass_floor;
+    }
+    if (removed) {
+        double rho_mb[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k)
+            rho_mb[k] = smax(safe_div(smax(m_before[k], 0.0), smax(alpha_full[k], inactive)), mass_floor);
+        const double kinetic_before = 0.5 * rho_before * v2_before;
+        // U.E is finite here, so finite_or(U.E, kinetic_before) == U.E
+        const double internal_before = safe_div(U.E - kinetic_before, rho_before);
+        const double pressure_before = smax(
+            pressure_from_internal_energy<M>(P, rho_before, internal_before, alpha_full, rho_mb), p_floor);
+#pragma unroll
+        for (int d = 0; d < D; ++d) U.mom[d] = rho_total * vel_before[d];
+        double rho_m[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k)
+            rho_m[k] = smax(safe_div(U.m[k], smax(alpha_full[k], inactive)), mass_floor);
+        const double ie = internal_energy_from_pressure<M>(P, rho_total, pressure_before, alpha_full, rho_m);
+        U.E = rho_total * ie + 0.5 * rho_total * v2_before;
+    }
+    double v2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const double v = safe_div(U.mom[d], rho_total);
+        v2 += v * v;
+    }
+    double rho_m[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) rho_m[k] = smax(safe_div(U.m[k], smax(alpha_full[k], inactive)), mass_floor);
+    const double kinetic = 0.5 * rho_total * v2;
+    const MixCoef<M> c = mix_coef<M>(P, alpha_full, rho_m);
+    const double minimum_internal = mix_energy<M>(P, c, p_floor);
+    U.E = smax(finite_or(U.E), kinetic + minimum_internal);
+}
+
+// ---- reconstruction (src/core/fv/limiters.hpp, src/dim/reconstruction/muscl.hpp)
+// limiters.hpp:9-15
+__device__ __forceinline__ double minmod(double a, double b)
+{
+    if (a * b <= 0.0) return 0.0;
+    return __builtin_fabs(a) < __builtin_fabs(b) ? a : b;
+}
+
+// muscl.hpp:26-40
+template<int D, int M>
+__device__ __forceinline__ void enforce_positive_primitive(Prim<D, M>& Q)
+{
+#pragma unroll
+    for (int k = 0; k < M; ++k) Q.rho[k] = smax(Q.rho[k], 1e-12);
+    sanitise_alpha<M>(Q.alpha, 0.0);
+    Q.p = smax(Q.p, 1e-12);
+}
+
+// muscl.hpp:43-112
+template<int D, int M>
+__device__ __forceinline__ void reconstruct_cell_faces(const Prim<D, M>& L, const Prim<D, M>& C,
+                                                       const Prim<D, M>& R, Prim<D, M>& lf,
+                                                       Prim<D, M>& rf)
+{
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const double s = minmod(C.rho[k] - L.rho[k], R.rho[k] - C.rho[k]);
+        lf.rho[k] = C.rho[k] - 0.5 * s;
+        rf.rho[k] = C.rho[k] + 0.5 * s;
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const double s = minmod(C.alpha[k] - L.alpha[k], R.alpha[k] - C.alpha[k]);
+        lf.alpha[k] = C.alpha[k] - 0.5 * s;
+        rf.alpha[k] = C.alpha[k] + 0.5 * s;
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const double s = minmod(C.vel[d] - L.vel[d], R.vel[d] - C.vel[d]);
+        lf.vel[d] = C.vel[d] - 0.5 * s;
+        rf.vel[d] = C.vel[d] + 0.5 * s;
+    }
+    const double s = minmod(C.p - L.p, R.p - C.p);
+    lf.p = C.p - 0.5 * s;
+    rf.p = C.p + 0.5 * s;
+    enforce_positive_primitive<D, M>(lf);
+    enforce_positive_primitive<D, M>(rf);
+}
+
+// dot(vel, e_dir) as the reference sums it (vector_ops.hpp:9-19): a running
+// sum from +0.0 over every component, so -0.0 becomes +0.0
+template<int D, int DIR>
+__device__ __forceinline__ double normal_velocity(const double* vel)
+{
+    double un = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) un += vel[d] * (d == DIR ? 1.0 : 0.0);
+    return un;
+}
+
+// flux.hpp:11-43 with normal = e_DIR
+template<int D, int M, int DIR>
+__device__ __forceinline__ void flux_normal(const State<D, M>& U, const Prim<D, M>& Q, Flux<D, M>& F)
+{
+    const double rho_total = total_density<D, M>(U);
+    const double un = normal_velocity<D, DIR>(Q.vel);
+#pragma unroll
+    for (int k = 0; k < M; ++k) F.m[k] = U.m[k] * un;
+#pragma unroll
+    for (int d = 0; d < D; ++d) F.mom[d] = rho_total * Q.vel[d] * un + Q.p * (d == DIR ? 1.0 : 0.0);
+    F.E = (U.E + Q.p) * un;
+}
+
+// muscl.hpp:143-191
+template<int D, int M, int DIR>
+__device__ __forceinline__ void hancock_predict(const EosParams& P, const State<D, M>& Uc,
+                                                const Prim<D, M>& Pc, const State<D, M>& ULf,
+                                                const State<D, M>& URf, const Prim<D, M>& PLf,
+                                                const Prim<D, M>& PRf, double dt, double dx,
+                                                State<D, M>& out)
+{
+    Flux<D, M> FL, FR;
+    flux_normal<D, M, DIR>(ULf, PLf, FL);
+    flux_normal<D, M, DIR>(URf, PRf, FR);
+    const double factor = 0.5 * dt / dx;
+    out = Uc;
+#pragma unroll
+    for (int k = 0; k < M; ++k) out.m[k] -= factor * (FR.m[k] - FL.m[k]);
+#pragma unroll
+    for (int d = 0; d < D; ++d) out.mom[d] -= factor * (FR.mom[d] - FL.mom[d]);
+    out.E -= factor * (FR.E - FL.E);
+    const double u_left = PLf.vel[DIR];
+    const double u_right = PRf.vel[DIR];
+    const double div_u = (u_right - u_left) / dx;
+    double rhs[M];
+    alpha_rhs_coefficients<D, M>(P, Pc, rhs);
+    double an[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        an[k] = Pc.alpha[k];
+        an[k] -= factor * (PRf.alpha[k] * u_right - PLf.alpha[k] * u_left);
+        an[k] += 0.5 * dt * rhs[k] * div_u;
+    }
+    sanitise_alpha<M>(an, 0.0);
+#pragma unroll
+    for (int k = 0; k < M - 1; ++k) out.a[k] = an[k];
+    repair_state<D, M>(P, out);
+}
+
+// ---- HLLC (src/dim/riemann/hllc.hpp:15-195) ------------------------------
+// Returns false on the reference's throwing branch (non-finite density or
+// pressure in require_positive, numerical_safety.hpp:67-78).
+template<int D, int M, int DIR>
+__device__ __forceinline__ bool hllc_flux(const EosParams& P, const State<D, M>& UL,
+                                          const State<D, M>& UR, const Prim<D, M>& PL,
+                                          const Prim<D, M>& PR, Flux<D, M>& F, double& fv)
+{
+    double rhoL = total_density<D, M>(UL);
+    double rhoR = total_density<D, M>(UR);
+    double pL = PL.p, pR = PR.p;
+    if (!finite(rhoL) || !finite(rhoR) || !finite(pL) || !finite(pR)) return false;
+    if (rhoL <= 0.0) rhoL = 1e-10;
+    if (rhoR <= 0.0) rhoR = 1e-10;
+    if (pL <= 0.0) pL = 1e-10;
+    if (pR <= 0.0) pR = 1e-10;
+    const double unL = normal_velocity<D, DIR>(PL.vel);
+    const double unR = normal_velocity<D, DIR>(PR.vel);
+    const double cL = mixture_sound_speed<M>(P, rhoL, pL, PL.alpha, PL.rho);
+    const double cR = mixture_sound_speed<M>(P, rhoR, pR, PR.alpha, PR.rho);
+    const double sL = smin(unL - cL, unR - cR);
+    const double sR = smax(unL + cL, unR + cR);
+    const double numerator = (pR - pL) + rhoL * unL * (sL - unL) - rhoR * unR * (sR - unR);
+    const double denom = rhoL * (sL - unL) - rhoR * (sR - unR);
+    double s_star = safe_div(numerator, denom);
+    if (__builtin_fabs(pL - pR) < 1e-10 && __builtin_fabs(unL - unR) < 1e-10) s_star = unL;
+
+    Flux<D, M> FL, FR;
+    flux_normal<D, M, DIR>(UL, PL, FL);
+    flux_normal<D, M, DIR>(UR, PR, FR);
+    const bool average = !finite(sL) || !finite(sR) || !finite(s_star) ||
+                         (!(0.0 <= sL) && !(sR <= 0.0) && !(sL < sR));
+    if (average) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) F.m[k] = 0.5 * (FL.m[k] + FR.m[k]);
+#pragma unroll
+        for (int d = 0; d < D; ++d) F.mom[d] = 0.5 * (FL.mom[d] + FR.mom[d]);
+        F.E = 0.5 * (FL.E + FR.E);
+        fv = 0.5 * (unL + unR);
+        return true;
+    }
+    if (0.0 <= sL) {
+        F = FL;
+        fv = unL;
+        return true;
+    }
+    if (sR <= 0.0) {
+        F = FR;
+        fv = unR;
+        return true;
+    }
+    const bool left = (0.0 <= s_star);  // sL < 0 here
+    const State<D, M>& UK = left ? UL : UR;
+    const Prim<D, M>& PK = left ? PL : PR;
+    const Flux<D, M>& FK = left ? FL : FR;
+    const double rhoK = left ? rhoL : rhoR;
+    const double pK = left ? pL : pR;
+    const double sK = left ? sL : sR;
+    const double unK = left ? unL : unR;
+    const double rho_star = rhoK * safe_div(sK - unK, sK - s_star);
+    const double E_star =
+        rho_star * (safe_div(UK.E, rhoK) + (s_star - unK) * (s_star + safe_div(pK, rhoK * (sK - unK))));
+    fv = s_star;
+    if (rho_star <= 0.0 || !finite(rho_star) || !finite(E_star)) {
+        // hll_flux (hllc.hpp:47-59): (sR FL - sL FR + sL sR (UR-UL)) * (1/(sR-sL))
+        const double inv = 1.0 / (sR - sL);
+        const double sLsR = sL * sR;
+#pragma unroll
+        for (int k = 0; k < M; ++k) F.m[k] = inv * ((sR * FL.m[k] - sL * FR.m[k]) + sLsR * (UR.m[k] - UL.m[k]));
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            F.mom[d] = inv * ((sR * FL.mom[d] - sL * FR.mom[d]) + sLsR * (UR.mom[d] - UL.mom[d]));
+        F.E = inv * ((sR * FL.E - sL * FR.E) + sLsR * (UR.E - UL.E));
+        return true;
+    }
+    const double mass_scale = safe_div(rho_star, rhoK);
+    const double delta = s_star - unK;
+#pragma unroll
+    for (int k = 0; k < M; ++k) F.m[k] = FK.m[k] + sK * (UK.m[k] * mass_scale - UK.m[k]);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        double u_star = PK.vel[d];
+        u_star += delta * (d == DIR ? 1.0 : 0.0);
+        F.mom[d] = FK.mom[d] + sK * (rho_star * u_star - UK.mom[d]);
+    }
+    F.E = FK.E + sK * (E_star - UK.E);
+    return true;
+}
+
+}  // namespace dimhip

@@ -1,0 +1,845 @@
+// dimhip.hip — the C ABI of include/dimhip.h on top of the gfx950 kernels.
+//
+// One context = one grid, one HIP stream, SoA state double-buffered in HBM
+// plus a device-resident step-control block (CFL minimum slots, dt, time).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/dimhip.h"
+#include "kernels_fp64.hpp"
+#include "kernels_fp32.hpp"
+
+using namespace dimhip;
+
+namespace {
+
+thread_local std::string g_global_err;
+
+constexpr int kSweepL0 = 128, kSweepW0 = 2;   // x sweep: 128 positions x 2 lines
+constexpr int kSweepL1 = 64, kSweepW1 = 8;    // y/z sweep: 8 lines (x) x 64 positions
+
+enum KernelClass { KC_SWEEP = 0, KC_BC = 1, KC_CFL = 2, KC_COUNT = 3 };
+
+struct Ops;
+
+}  // namespace
+
+struct dimhip_ctx {
+    dimhip_desc desc{};
+    Grid grid{};
+    EosParams eos{};
+    EosParamsF eosf{};
+    BcArgs bc{};
+    int nv = 0;
+    hipStream_t stream = nullptr;
+    double* q[2] = {nullptr, nullptr};
+    double* qacc = nullptr;
+    float* qf[2] = {nullptr, nullptr};
+    float* qfacc = nullptr;
+    int cur = 0;
+    StepCtl* ctl = nullptr;
+    StepCtl* ctl_host = nullptr;  // pinned
+    double* staging = nullptr;
+    long long staging_cells = 0;
+    double* partial = nullptr;
+    double* totals = nullptr;
+    int partial_blocks = 0;
+    bool dt_ready = false;       // ctl->dtmin[slot] holds the CFL minimum of the current state
+    double dt_ready_cfl = 0.0;   // the cfl it was computed with
+    std::string err;
+    const Ops* ops = nullptr;
+    // graph for dimhip_advance
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    double graph_cfl = -1.0;
+    int graph_cur = -1;
+    // event timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_pending;  // (class, index of start event)
+    size_t ev_next = 0;
+    double kt_sum[KC_COUNT] = {0, 0, 0};
+    long long kt_cnt[KC_COUNT] = {0, 0, 0};
+};
+
+namespace {
+
+int fail(dimhip_ctx* ctx, const std::string& msg)
+{
+    if (ctx) ctx->err = msg;
+    g_global_err = msg;
+    return 1;
+}
+
+#define HIP_OK(ctx, expr)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail((ctx), std::string("HIP error: ") + hipGetErrorString(e_) +    \
+                                   " at " #expr);                                      \
+    } while (0)
+
+// ---- event timing ---------------------------------------------------------
+int ev_begin(dimhip_ctx* c)
+{
+    if (!c->timing) return -1;
+    if (c->ev_next + 2 > c->ev_pool.size()) {
+        for (int k = 0; k < 64; ++k) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return -1;
+            c->ev_pool.push_back(e);
+        }
+    }
+    const int i = (int)c->ev_next;
+    c->ev_next += 2;
+    hipEventRecord(c->ev_pool[i], c->stream);
+    return i;
+}
+
+void ev_end(dimhip_ctx* c, int i, int cls)
+{
+    if (i < 0) return;
+    hipEventRecord(c->ev_pool[i + 1], c->stream);
+    c->ev_pending.push_back({cls, i});
+}
+
+void ev_collect(dimhip_ctx* c)
+{
+    for (auto& pr : c->ev_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev_pool[pr.second], c->ev_pool[pr.second + 1]) == hipSuccess) {
+            c->kt_sum[pr.first] += ms;
+            c->kt_cnt[pr.first] += 1;
+        }
+    }
+    c->ev_pending.clear();
+    c->ev_next = 0;
+}
+
+// ---- per-(D, M) launch table ---------------------------------------------
+struct Ops {
+    virtual ~Ops() = default;
+    virtual void sweep(dimhip_ctx* c, int dir, int mode, bool fuse, const void* in, void* out) const = 0;
+    virtual void bc(dimhip_ctx* c, void* q, int end_step) const = 0;
+    virtual void cfl(dimhip_ctx* c, const void* q, bool unsplit, double cfl) const = 0;
+    virtual void totals(dimhip_ctx* c, const void* q) const = 0;
+    virtual int kat(const EosParams& P, int which, int dir, int count, const double* in0, const double* in1,
+                    double* out0, double* out1, int* err, hipStream_t s) const = 0;
+};
+
+CflArgs make_cfl_args(const dimhip_ctx* c, double cfl, bool unsplit)
+{
+    CflArgs a{};
+    for (int d = 0; d < 3; ++d) {
+        a.dx[d] = c->grid.dx[d];
+        a.cfl_dx[d] = cfl * c->grid.dx[d];
+    }
+    a.cfl = cfl;
+    a.unsplit = unsplit ? 1 : 0;
+    return a;
+}
+
+int grid_blocks(long long work, int per)
+{
+    long long b = (work + per - 1) / per;
+    if (b < 1) b = 1;
+    if (b > 8192) b = 8192;
+    return (int)b;
+}
+
+template<int D, int M>
+struct OpsImpl final : Ops {
+    template<int DIR, int MODE, bool FUSE>
+    static void launch_sweep64(dimhip_ctx* c, const double* in, double* out, const CflArgs& ca)
+    {
+        const Grid& g = c->grid;
+        if constexpr (DIR == 0) {
+            constexpr int T = kSweepL0 - 6;
+            dim3 blk(kSweepL0, kSweepW0, 1);
+            dim3 grd((g.N[0] + T - 1) / T, (g.N[1] + kSweepW0 - 1) / kSweepW0, g.N[2]);
+            sweep64_kernel<D, M, 0, kSweepL0, kSweepW0, MODE, FUSE><<<grd, blk, 0, c->stream>>>(in, out, g, c->eos, c->ctl, ca);
+        } else {
+            constexpr int T = kSweepL1 - 6;
+            dim3 blk(kSweepW1, kSweepL1, 1);
+            dim3 grd((g.N[0] + kSweepW1 - 1) / kSweepW1, (g.N[DIR] + T - 1) / T, DIR == 1 ? g.N[2] : g.N[1]);
+            sweep64_kernel<D, M, DIR, kSweepL1, kSweepW1, MODE, FUSE><<<grd, blk, 0, c->stream>>>(in, out, g, c->eos, c->ctl, ca);
+        }
+    }
+
+    template<int DIR>
+    static void sweep_dir(dimhip_ctx* c, int mode, bool fuse, const void* in, void* out)
+    {
+        const CflArgs ca = make_cfl_args(c, c->dt_ready_cfl, c->desc.time_update == DIMHIP_UNSPLIT);
+        if (c->desc.precision == DIMHIP_FP64) {
+            const double* qi = (const double*)in;
+            double* qo = (double*)out;
+            if (mode == 0 && !fuse) launch_sweep64<DIR, 0, false>(c, qi, qo, ca);
+            else if (mode == 0 && fuse) launch_sweep64<DIR, 0, true>(c, qi, qo, ca);
+            else if (mode == 1 && !fuse) launch_sweep64<DIR, 1, false>(c, qi, qo, ca);
+            else launch_sweep64<DIR, 1, true>(c, qi, qo, ca);
+        } else {
+            launch_sweep32<D, M, DIR>(c->stream, c->grid, c->eosf, c->eos, c->ctl, ca, mode, fuse,
+                                     (const float*)in, (float*)out);
+        }
+    }
+
+    void sweep(dimhip_ctx* c, int dir, int mode, bool fuse, const void* in, void* out) const override
+    {
+        if (dir == 0) sweep_dir<0>(c, mode, fuse, in, out);
+        if constexpr (D >= 2)
+            if (dir == 1) sweep_dir<1>(c, mode, fuse, in, out);
+        if constexpr (D >= 3)
+            if (dir == 2) sweep_dir<2>(c, mode, fuse, in, out);
+    }
+
+    void bc(dimhip_ctx* c, void* q, int end_step) const override
+    {
+        const long long total = c->bc.face_start[2 * D];
+        const int blocks = grid_blocks(total, 256);
+        if (c->desc.precision == DIMHIP_FP64)
+            bc64_kernel<D, M><<<blocks, 256, 0, c->stream>>>((double*)q, c->grid, c->eos, c->bc, c->ctl, end_step);
+        else
+            bc32_kernel<D, M><<<blocks, 256, 0, c->stream>>>((float*)q, c->grid, c->eosf, c->eos, c->bc, c->ctl, end_step);
+    }
+
+    void cfl(dimhip_ctx* c, const void* q, bool unsplit, double cfl) const override
+    {
+        const CflArgs ca = make_cfl_args(c, cfl, unsplit);
+        const int blocks = grid_blocks(c->grid.ncells, 256 * 4);
+        if (c->desc.precision == DIMHIP_FP64)
+            cfl64_kernel<D, M><<<blocks, 256, 0, c->stream>>>((const double*)q, c->grid, c->eos, c->ctl, ca);
+        else
+            cfl32_kernel<D, M><<<blocks, 256, 0, c->stream>>>((const float*)q, c->grid, c->eos, c->ctl, ca);
+    }
+
+    void totals(dimhip_ctx* c, const void* q) const override
+    {
+        double volume = 1.0;
+        for (int d = 0; d < D; ++d) volume *= c->grid.dx[d];
+        const long long chunk = (c->grid.ncells + c->partial_blocks - 1) / c->partial_blocks;
+        if (c->desc.precision == DIMHIP_FP64)
+            cons_totals_kernel<D, M><<<c->partial_blocks, 256, 0, c->stream>>>((const double*)q, c->grid, volume, chunk, c->partial);
+        else
+            cons_totals32_kernel<D, M><<<c->partial_blocks, 256, 0, c->stream>>>((const float*)q, c->grid, volume, chunk, c->partial);
+        sum_partials_kernel<<<1, 64, 0, c->stream>>>(c->partial, c->partial_blocks, M + 1 + D + 1, c->totals);
+    }
+
+    int kat(const EosParams& P, int which, int dir, int count, const double* in0, const double* in1, double* out0,
+            double* out1, int* err, hipStream_t s) const override
+    {
+        const int blocks = (count + 127) / 128;
+        if (dir == 0) {
+            kat64_kernel<D, M, 0><<<blocks, 128, 0, s>>>(P, which, count, in0, in1, out0, out1, err);
+            return 0;
+        }
+        if constexpr (D >= 2)
+            if (dir == 1) {
+                kat64_kernel<D, M, 1><<<blocks, 128, 0, s>>>(P, which, count, in0, in1, out0, out1, err);
+                return 0;
+            }
+        if constexpr (D >= 3)
+            if (dir == 2) {
+                kat64_kernel<D, M, 2><<<blocks, 128, 0, s>>>(P, which, count, in0, in1, out0, out1, err);
+                return 0;
+            }
+        return 1;
+    }
+};
+
+const Ops* select_ops(int dim, int nmat)
+{
+    static const OpsImpl<1, 2> o12;
+    static const OpsImpl<2, 2> o22;
+    static const OpsImpl<3, 2> o32;
+    static const OpsImpl<1, 1> o11;
+    static const OpsImpl<2, 1> o21;
+    static const OpsImpl<3, 1> o31;
+    static const OpsImpl<1, 3> o13;
+    static const OpsImpl<2, 3> o23;
+    static const OpsImpl<3, 3> o33;
+    if (nmat == 2) return dim == 1 ? (const Ops*)&o12 : dim == 2 ? (const Ops*)&o22 : (const Ops*)&o32;
+    if (nmat == 1) return dim == 1 ? (const Ops*)&o11 : dim == 2 ? (const Ops*)&o21 : (const Ops*)&o31;
+    if (nmat == 3) return dim == 1 ? (const Ops*)&o13 : dim == 2 ? (const Ops*)&o23 : (const Ops*)&o33;
+    return nullptr;
+}
+
+int validate_desc(const dimhip_desc* d, std::string& msg)
+{
+    if (d->dim < 1 || d->dim > 3) return msg = "dimhip: dim must be 1, 2 or 3", 1;
+    if (d->nmat < 1 || d->nmat > 3) return msg = "dimhip: nmat must be 1, 2 or 3", 1;
+    for (int k = 0; k < d->nmat; ++k) {
+        if (!(d->gamma[k] > 1.0)) return msg = "dim::EOSParams: gamma must be > 1", 1;
+        if (d->eos_kind[k] != DIMHIP_EOS_IDEAL && d->eos_kind[k] != DIMHIP_EOS_STIFFENED)
+            return msg = "dimhip: unsupported EOS kind", 1;
+    }
+    for (int a = 0; a < d->dim; ++a) {
+        if (d->N[a] <= 0) return msg = "dim::total_cells: grid size must be positive", 1;
+        if (d->N[a] == 2) return msg = "dimhip: grid extent 2 is not supported (boundary sources overlap)", 1;
+        if (!(d->dx[a] > 0.0)) return msg = "dim::compute_dt_cfl: grid spacing must be positive", 1;
+        for (int side = 0; side < 2; ++side) {
+            const int b = side ? d->bc_hi[a] : d->bc_lo[a];
+            if (b < 0 || b > 3) return msg = "Invalid boundary condition", 1;
+            if (b == DIMHIP_BC_NONREFLECTIVE && d->N[a] < 4)
+                return msg = "dimhip: nonreflective boundaries need at least 4 cells", 1;
+        }
+    }
+    if (d->lambda_model < 0 || d->lambda_model > 2) return msg = "dim::alpha_rhs_coefficients: unknown lambda model", 1;
+    if (d->time_update != DIMHIP_SPLIT && d->time_update != DIMHIP_UNSPLIT) return msg = "dimhip: unknown time_update", 1;
+    if (d->precision != DIMHIP_FP64) return msg = "dimhip: unknown precision", 1;
+    return 0;
+}
+
+__global__ void init_ctl_kernel(StepCtl* ctl)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->dtmin[0] = kInfBits;
+        ctl->dtmin[1] = kInfBits;
+        ctl->time = 0.0;
+        ctl->dt = 0.0;
+        ctl->dt_max = 0.0;
+        ctl->t_end = 0.0;
+        ctl->step = 0;
+        ctl->slot = 0;
+        ctl->active = 1;
+        ctl->use_t_end = 0;
+        ctl->err = 0;
+    }
+}
+
+__global__ void set_mode_kernel(StepCtl* ctl, int use_t_end, double t_end, double time, int set_time)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->use_t_end = use_t_end;
+        ctl->t_end = t_end;
+        if (set_time) ctl->time = time;
+        ctl->err = 0;
+    }
+}
+
+__global__ void copy_kernel(const double* __restrict__ a, double* __restrict__ b, long long n)
+{
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        b[t] = a[t];
+}
+
+__global__ void copy32_kernel(const float* __restrict__ a, float* __restrict__ b, long long n)
+{
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        b[t] = a[t];
+}
+
+void* state_ptr(dimhip_ctx* c, int which)
+{
+    return c->desc.precision == DIMHIP_FP64 ? (void*)c->q[which] : (void*)c->qf[which];
+}
+
+void* acc_ptr(dimhip_ctx* c) { return c->desc.precision == DIMHIP_FP64 ? (void*)c->qacc : (void*)c->qfacc; }
+
+void copy_state(dimhip_ctx* c, const void* a, void* b)
+{
+    const long long n = c->grid.pitch * c->nv;
+    if (c->desc.precision == DIMHIP_FP64)
+        copy_kernel<<<grid_blocks(n, 1024), 256, 0, c->stream>>>((const double*)a, (double*)b, n);
+    else
+        copy32_kernel<<<grid_blocks(n, 1024), 256, 0, c->stream>>>((const float*)a, (float*)b, n);
+}
+
+bool fusable(const dimhip_ctx* c)
+{
+    if (c->desc.time_update != DIMHIP_SPLIT) return false;
+    for (int d = 0; d < c->desc.dim; ++d)
+        if (c->desc.bc_lo[d] == DIMHIP_BC_NONREFLECTIVE || c->desc.bc_hi[d] == DIMHIP_BC_NONREFLECTIVE) return false;
+    return true;
+}
+
+// Enqueue one advance_one_step.  When dt_max_host < 0 the step takes
+// dt_max = t_end - time on the device.
+void enqueue_step(dimhip_ctx* c, double cfl, double dt_max_host)
+{
+    const bool unsplit = c->desc.time_update == DIMHIP_UNSPLIT;
+    const int D = c->desc.dim;
+    if (!c->dt_ready || c->dt_ready_cfl != cfl) {
+        const int e = ev_begin(c);
+        reset_dtmin_kernel<<<1, 1, 0, c->stream>>>(c->ctl, 0);
+        c->ops->cfl(c, state_ptr(c, c->cur), unsplit, cfl);
+        ev_end(c, e, KC_CFL);
+    }
+    begin_step_kernel<<<1, 1, 0, c->stream>>>(c->ctl, dt_max_host);
+    const bool fuse = fusable(c);
+    c->dt_ready_cfl = cfl;
+    if (!unsplit) {
+        for (int dir = 0; dir < D; ++dir) {
+            void* in = state_ptr(c, c->cur);
+            void* out = state_ptr(c, c->cur ^ 1);
+            int e = ev_begin(c);
+            c->ops->sweep(c, dir, 0, fuse && dir == D - 1, in, out);
+            ev_end(c, e, KC_SWEEP);
+            e = ev_begin(c);
+            c->ops->bc(c, out, dir == D - 1);
+            ev_end(c, e, KC_BC);
+            c->cur ^= 1;
+        }
+    } else {
+        void* base = state_ptr(c, c->cur);
+        void* acc = state_ptr(c, c->cur ^ 1);
+        copy_state(c, base, acc);
+        for (int dir = 0; dir < D; ++dir) {
+            const int e = ev_begin(c);
+            c->ops->sweep(c, dir, 1, false, base, acc);
+            ev_end(c, e, KC_SWEEP);
+        }
+        const int e = ev_begin(c);
+        c->ops->bc(c, acc, 1);
+        ev_end(c, e, KC_BC);
+        c->cur ^= 1;
+    }
+    c->dt_ready = fuse;
+}
+
+int check_device_error(dimhip_ctx* c)
+{
+    if (c->ctl_host->err & ERR_NONFINITE_HLLC) return fail(c, "dim::hllc_flux_normal: invalid left density (non-finite)");
+    if (c->ctl_host->err & ERR_NONPOSITIVE_DT) return fail(c, "run_dim_case: non-positive timestep");
+    return 0;
+}
+
+int fetch_ctl(dimhip_ctx* c)
+{
+    HIP_OK(c, hipMemcpyAsync(c->ctl_host, c->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    ev_collect(c);
+    return 0;
+}
+
+void drop_graph(dimhip_ctx* c)
+{
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    c->graph_cfl = -1.0;
+    c->graph_cur = -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dimhip_version(void) { return DIMHIP_VERSION; }
+
+const char* dimhip_last_error(const dimhip_ctx* ctx) { return ctx ? ctx->err.c_str() : g_global_err.c_str(); }
+
+int dimhip_create(const dimhip_desc* desc, dimhip_ctx** out)
+{
+    if (!desc || !out) return fail(nullptr, "dimhip_create: null argument");
+    std::string msg;
+    if (validate_desc(desc, msg)) return fail(nullptr, msg);
+    auto c = std::make_unique<dimhip_ctx>();
+    c->desc = *desc;
+    for (int d = desc->dim; d < 3; ++d) {
+        c->desc.N[d] = 1;
+        c->desc.dx[d] = 1.0;
+    }
+    c->ops = select_ops(desc->dim, desc->nmat);
+    if (!c->ops) return fail(nullptr, "dimhip: no kernels for this dim/nmat");
+    HIP_OK(nullptr, hipSetDevice(desc->device));
+    Grid& g = c->grid;
+    for (int d = 0; d < 3; ++d) {
+        g.N[d] = c->desc.N[d];
+        g.dx[d] = c->desc.dx[d];
+    }
+    g.stride[0] = 1;
+    g.stride[1] = g.N[0];
+    g.stride[2] = (long long)g.N[0] * g.N[1];
+    g.ncells = (long long)g.N[0] * g.N[1] * g.N[2];
+    g.pitch = (g.ncells + 63) / 64 * 64;
+    c->nv = 2 * desc->nmat + desc->dim;
+    for (int k = 0; k < kMaxMat; ++k) {
+        const bool on = k < desc->nmat;
+        c->eos.kind[k] = on ? desc->eos_kind[k] : 0;
+        c->eos.gamma[k] = on ? desc->gamma[k] : 1.4;
+        c->eos.pinf[k] = on ? desc->p_inf[k] : 0.0;
+        c->eos.gm1[k] = c->eos.gamma[k] - 1.0;
+        c->eos.gpinf[k] = c->eos.gamma[k] * c->eos.pinf[k];
+        c->eosf.kind[k] = c->eos.kind[k];
+        c->eosf.gamma[k] = (float)c->eos.gamma[k];
+        c->eosf.pinf[k] = (float)c->eos.pinf[k];
+        c->eosf.gm1[k] = (float)c->eos.gm1[k];
+        c->eosf.gpinf[k] = (float)c->eos.gpinf[k];
+        c->eosf.inv_gm1[k] = (float)(1.0 / c->eos.gm1[k]);
+    }
+    c->eos.lambda_model = desc->lambda_model;
+    c->eos.alpha_source_floor = desc->alpha_source_floor;
+    c->eosf.lambda_model = desc->lambda_model;
+    c->eosf.alpha_source_floor = (float)desc->alpha_source_floor;
+    // ring cells per face (d, side), owned by the face of their lowest boundary axis
+    long long acc = 0;
+    for (int f = 0; f < 6; ++f) {
+        c->bc.face_start[f] = acc;
+        const int fd = f >> 1;
+        if (fd >= desc->dim || g.N[fd] <= 1) continue;
+        long long cnt = 1;
+        for (int d = 0; d < desc->dim; ++d) {
+            if (d == fd) continue;
+            if (d < fd && g.N[d] > 1) cnt *= (g.N[d] - 2);
+            else cnt *= g.N[d];
+        }
+        acc += cnt;
+    }
+    c->bc.face_start[6] = acc;
+    for (int f = 2 * desc->dim; f <= 6; ++f) c->bc.face_start[f] = acc;
+    for (int d = 0; d < 3; ++d) {
+        c->bc.bc_lo[d] = d < desc->dim ? desc->bc_lo[d] : 0;
+        c->bc.bc_hi[d] = d < desc->dim ? desc->bc_hi[d] : 0;
+    }
+    HIP_OK(nullptr, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    const size_t bytes = (size_t)g.pitch * c->nv * (desc->precision == DIMHIP_FP64 ? sizeof(double) : sizeof(float));
+    if (desc->precision == DIMHIP_FP64) {
+        HIP_OK(nullptr, hipMalloc(&c->q[0], bytes));
+        HIP_OK(nullptr, hipMalloc(&c->q[1], bytes));
+    } else {
+        HIP_OK(nullptr, hipMalloc(&c->qf[0], bytes));
+        HIP_OK(nullptr, hipMalloc(&c->qf[1], bytes));
+    }
+    HIP_OK(nullptr, hipMemsetAsync(state_ptr(c.get(), 0), 0, bytes, c->stream));
+    HIP_OK(nullptr, hipMemsetAsync(state_ptr(c.get(), 1), 0, bytes, c->stream));
+    HIP_OK(nullptr, hipMalloc(&c->ctl, sizeof(StepCtl)));
+    HIP_OK(nullptr, hipHostMalloc(&c->ctl_host, sizeof(StepCtl), hipHostMallocDefault));
+    c->partial_blocks = (int)std::min<long long>(1024, std::max<long long>(1, g.ncells / 4096));
+    HIP_OK(nullptr, hipMalloc(&c->partial, sizeof(double) * (size_t)c->partial_blocks * 16));
+    HIP_OK(nullptr, hipMalloc(&c->totals, sizeof(double) * 16));
+    init_ctl_kernel<<<1, 1, 0, c->stream>>>(c->ctl);
+    HIP_OK(nullptr, hipGetLastError());
+    HIP_OK(nullptr, hipStreamSynchronize(c->stream));
+    *out = c.release();
+    return 0;
+}
+
+int dimhip_destroy(dimhip_ctx* c)
+{
+    if (!c) return 0;
+    hipSetDevice(c->desc.device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    for (auto e : c->ev_pool) hipEventDestroy(e);
+    hipFree(c->q[0]);
+    hipFree(c->q[1]);
+    hipFree(c->qacc);
+    hipFree(c->qf[0]);
+    hipFree(c->qf[1]);
+    hipFree(c->qfacc);
+    hipFree(c->ctl);
+    hipHostFree(c->ctl_host);
+    hipFree(c->staging);
+    hipFree(c->partial);
+    hipFree(c->totals);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int dimhip_upload(dimhip_ctx* c, const double* packed, size_t ncells)
+{
+    if (!c || !packed) return fail(c, "dimhip_upload: null argument");
+    if ((long long)ncells != c->grid.ncells) return fail(c, "dim::advance_one_step: grid size mismatch");
+    HIP_OK(c, hipSetDevice(c->desc.device));
+    const long long chunk = std::min<long long>(c->grid.ncells, 1ll << 22);
+    if (c->staging_cells < chunk) {
+        hipFree(c->staging);
+        c->staging = nullptr;
+        HIP_OK(c, hipMalloc(&c->staging, sizeof(double) * (size_t)chunk * c->nv));
+        c->staging_cells = chunk;
+    }
+    for (long long first = 0; first < c->grid.ncells; first += chunk) {
+        const long long cnt = std::min(chunk, c->grid.ncells - first);
+        HIP_OK(c, hipMemcpyAsync(c->staging, packed + first * c->nv, sizeof(double) * (size_t)(cnt * c->nv),
+                                 hipMemcpyHostToDevice, c->stream));
+        if (c->desc.precision == DIMHIP_FP64)
+            aos_to_soa_kernel<<<grid_blocks(cnt * c->nv, 1024), 256, 0, c->stream>>>(c->staging, c->q[c->cur], first, cnt, c->nv, c->grid.pitch);
+        else
+            aos_to_soa32_kernel<<<grid_blocks(cnt * c->nv, 1024), 256, 0, c->stream>>>(c->staging, c->qf[c->cur], first, cnt, c->nv, c->grid.pitch);
+        HIP_OK(c, hipGetLastError());
+        HIP_OK(c, hipStreamSynchronize(c->stream));
+    }
+    c->dt_ready = false;
+    return 0;
+}
+
+int dimhip_download(dimhip_ctx* c, double* packed, size_t ncells)
+{
+    if (!c || !packed) return fail(c, "dimhip_download: null argument");
+    if ((long long)ncells != c->grid.ncells) return fail(c, "dim::advance_one_step: grid size mismatch");
+    HIP_OK(c, hipSetDevice(c->desc.device));
+    const long long chunk = std::min<long long>(c->grid.ncells, 1ll << 22);
+    if (c->staging_cells < chunk) {
+        hipFree(c->staging);
+        c->staging = nullptr;
+        HIP_OK(c, hipMalloc(&c->staging, sizeof(double) * (size_t)chunk * c->nv));
+        c->staging_cells = chunk;
+    }
+    for (long long first = 0; first < c->grid.ncells; first += chunk) {
+        const long long cnt = std::min(chunk, c->grid.ncells - first);
+        if (c->desc.precision == DIMHIP_FP64)
+            soa_to_aos_kernel<<<grid_blocks(cnt * c->nv, 1024), 256, 0, c->stream>>>(c->q[c->cur], c->staging, first, cnt, c->nv, c->grid.pitch);
+        else
+            soa32_to_aos_kernel<<<grid_blocks(cnt * c->nv, 1024), 256, 0, c->stream>>>(c->qf[c->cur], c->staging, first, cnt, c->nv, c->grid.pitch);
+        HIP_OK(c, hipGetLastError());
+        HIP_OK(c, hipMemcpyAsync(packed + first * c->nv, c->staging, sizeof(double) * (size_t)(cnt * c->nv),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(c, hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+int dimhip_step(dimhip_ctx* c, double cfl, double dt_max, double* dt_out, dimhip_timings* timings)
+{
+    if (!c) return fail(c, "dimhip_step: null context");
+    if (cfl <= 0.0) return fail(c, "dim::compute_dt_cfl: cfl must be positive");
+    if (dt_max <= 0.0) return fail(c, "dim::compute_dt_cfl: dt_max must be positive");
+    HIP_OK(c, hipSetDevice(c->desc.device));
+    set_mode_kernel<<<1, 1, 0, c->stream>>>(c->ctl, 0, 0.0, 0.0, 0);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timings) {
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        hipEventRecord(e0, c->stream);
+    }
+    enqueue_step(c, cfl, dt_max);
+    HIP_OK(c, hipGetLastError());
+    if (timings) hipEventRecord(e1, c->stream);
+    if (fetch_ctl(c)) return 1;
+    if (timings) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        timings->cfl_seconds = 0.0;
+        timings->sweep_seconds = ms * 1e-3;
+        timings->boundary_seconds = 0.0;
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+    }
+    if (check_device_error(c)) return 1;
+    if (dt_out) *dt_out = c->ctl_host->dt;
+    return 0;
+}
+
+int dimhip_advance(dimhip_ctx* c, int nsteps, double cfl, double t_end, int use_graph, double* time_inout,
+                   long long* steps_done)
+{
+    if (!c || !time_inout) return fail(c, "dimhip_advance: null argument");
+    if (cfl <= 0.0) return fail(c, "dim::compute_dt_cfl: cfl must be positive");
+    HIP_OK(c, hipSetDevice(c->desc.device));
+    set_mode_kernel<<<1, 1, 0, c->stream>>>(c->ctl, 1, t_end, *time_inout, 1);
+    HIP_OK(c, hipMemcpyAsync(c->ctl_host, c->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    const long long step0 = c->ctl_host->step;
+    if (nsteps > 0) {
+        // the first step computes the CFL minimum of the uploaded state
+        if (!c->dt_ready || c->dt_ready_cfl != cfl) {
+            enqueue_step(c, cfl, -1.0);
+            --nsteps;
+        }
+        const bool graphable = use_graph && !c->timing && c->dt_ready;
+        if (graphable) {
+            if (!c->graph_exec || c->graph_cfl != cfl || c->graph_cur != c->cur) {
+                drop_graph(c);
+                // one graph = two steps, so the double buffer returns to `cur`
+                HIP_OK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                const int cur0 = c->cur;
+                enqueue_step(c, cfl, -1.0);
+                enqueue_step(c, cfl, -1.0);
+                const bool round_trip = (c->cur == cur0);
+                HIP_OK(c, hipStreamEndCapture(c->stream, &c->graph));
+                HIP_OK(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+                c->graph_cfl = cfl;
+                c->graph_cur = c->cur;
+                if (!round_trip) return fail(c, "dimhip_advance: graph does not return to its buffer");
+            }
+            while (nsteps >= 2) {
+                HIP_OK(c, hipGraphLaunch(c->graph_exec, c->stream));
+                nsteps -= 2;
+            }
+        }
+        while (nsteps > 0) {
+            enqueue_step(c, cfl, -1.0);
+            --nsteps;
+        }
+        HIP_OK(c, hipGetLastError());
+    }
+    if (fetch_ctl(c)) return 1;
+    if (check_device_error(c)) return 1;
+    *time_inout = c->ctl_host->time;
+    if (steps_done) *steps_done = c->ctl_host->step - step0;
+    return 0;
+}
+
+int dimhip_compute_dt(dimhip_ctx* c, double cfl, double dt_max, double* dt_out)
+{
+    if (!c || !dt_out) return fail(c, "dimhip_compute_dt: null argument");
+    if (cfl <= 0.0) return fail(c, "dim::compute_dt_cfl: cfl must be positive");
+    if (dt_max <= 0.0) return fail(c, "dim::compute_dt_cfl: dt_max must be positive");
+    HIP_OK(c, hipSetDevice(c->desc.device));
+    reset_dtmin_kernel<<<1, 1, 0, c->stream>>>(c->ctl, 0);
+    c->ops->cfl(c, state_ptr(c, c->cur), c->desc.time_update == DIMHIP_UNSPLIT, cfl);
+    HIP_OK(c, hipGetLastError());
+    if (fetch_ctl(c)) return 1;
+    const double m = __builtin_bit_cast(double, c->ctl_host->dtmin[c->ctl_host->slot]);
+    *dt_out = (m < dt_max) ? m : dt_max;
+    c->dt_ready = true;
+    c->dt_ready_cfl = cfl;
+    return 0;
+}
+
+int dimhip_sweep(dimhip_ctx* c, int dir, double dt)
+{
+    if (!c) return fail(c, "dimhip_sweep: null context");
+    if (dir < 0 || dir >= c->desc.dim) return fail(c, "dimhip_sweep: direction out of range");
+    HIP_OK(c, hipSetDevice(c->desc.device));
+    set_active_kernel<<<1, 1, 0, c->stream>>>(c->ctl, dt);
+    c->ops->sweep(c, dir, 0, false, state_ptr(c, c->cur), state_ptr(c, c->cur ^ 1));
+    HIP_OK(c, hipGetLastError());
+    c->cur ^= 1;
+    c->dt_ready = false;
+    if (fetch_ctl(c)) return 1;
+    return check_device_error(c);
+}
+
+int dimhip_apply_bc(dimhip_ctx* c)
+{
+    if (!c) return fail(c, "dimhip_apply_bc: null context");
+    HIP_OK(c, hipSetDevice(c->desc.device));
+    set_active_kernel<<<1, 1, 0, c->stream>>>(c->ctl, 0.0);
+    c->ops->bc(c, state_ptr(c, c->cur), 0);
+    HIP_OK(c, hipGetLastError());
+    c->dt_ready = false;
+    return fetch_ctl(c);
+}
+
+int dimhip_conservation(dimhip_ctx* c, double* out)
+{
+    if (!c || !out) return fail(c, "dimhip_conservation: null argument");
+    HIP_OK(c, hipSetDevice(c->desc.device));
+    c->ops->totals(c, state_ptr(c, c->cur));
+    HIP_OK(c, hipGetLastError());
+    const int no = c->desc.nmat + 1 + c->desc.dim + 1;
+    HIP_OK(c, hipMemcpyAsync(out, c->totals, sizeof(double) * no, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dimhip_synchronize(dimhip_ctx* c)
+{
+    if (!c) return fail(c, "dimhip_synchronize: null context");
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    ev_collect(c);
+    return 0;
+}
+
+int dimhip_set_timing(dimhip_ctx* c, int enable)
+{
+    if (!c) return fail(c, "dimhip_set_timing: null context");
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    ev_collect(c);
+    c->timing = enable != 0;
+    for (int k = 0; k < KC_COUNT; ++k) {
+        c->kt_sum[k] = 0.0;
+        c->kt_cnt[k] = 0;
+    }
+    return 0;
+}
+
+int dimhip_kernel_times(dimhip_ctx* c, double* mean_ms, long long* counts)
+{
+    if (!c || !mean_ms) return fail(c, "dimhip_kernel_times: null argument");
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    ev_collect(c);
+    for (int k = 0; k < KC_COUNT; ++k) {
+        mean_ms[k] = c->kt_cnt[k] ? c->kt_sum[k] / (double)c->kt_cnt[k] : 0.0;
+        if (counts) counts[k] = c->kt_cnt[k];
+    }
+    return 0;
+}
+
+int dimhip_kat(const dimhip_desc* desc, int which, int dir, int count, const double* in0, const double* in1,
+               double* out0, double* out1)
+{
+    if (!desc || count < 0) return fail(nullptr, "dimhip_kat: bad argument");
+    if (count == 0) return 0;
+    std::string msg;
+    dimhip_desc d = *desc;
+    for (int a = 0; a < d.dim; ++a) {
+        if (d.N[a] <= 0 || d.N[a] == 2) d.N[a] = 3;
+        if (!(d.dx[a] > 0.0)) d.dx[a] = 1.0;
+    }
+    if (validate_desc(&d, msg)) return fail(nullptr, msg);
+    const int M = d.nmat, D = d.dim;
+    const int nv = 2 * M + D, np = 2 * M + D + 1, nf = M + D + 1;
+    size_t n_in0 = 0, n_in1 = 0, n_out0 = 0, n_out1 = 0;
+    switch (which) {
+        case 0: n_in0 = (size_t)count * nv; n_out0 = (size_t)count * np; break;
+        case 1: n_in0 = (size_t)count * np; n_out0 = (size_t)count * nv; break;
+        case 2: n_in0 = (size_t)count * nv; n_out0 = (size_t)count * nv; break;
+        case 3: n_in0 = n_in1 = (size_t)count * nv; n_out0 = (size_t)count * nf; n_out1 = count; break;
+        case 4: n_in0 = (size_t)count * np; n_out0 = (size_t)count * M; break;
+        case 5: n_in0 = (size_t)count * 2; n_out0 = n_out1 = count; break;
+        default: return fail(nullptr, "dimhip_kat: unknown function");
+    }
+    if (which == 3 && (dir < 0 || dir >= D)) return fail(nullptr, "dimhip_kat: direction out of range");
+    HIP_OK(nullptr, hipSetDevice(d.device));
+    EosParams P{};
+    for (int k = 0; k < kMaxMat; ++k) {
+        const bool on = k < M;
+        P.kind[k] = on ? d.eos_kind[k] : 0;
+        P.gamma[k] = on ? d.gamma[k] : 1.4;
+        P.pinf[k] = on ? d.p_inf[k] : 0.0;
+        P.gm1[k] = P.gamma[k] - 1.0;
+        P.gpinf[k] = P.gamma[k] * P.pinf[k];
+    }
+    P.lambda_model = d.lambda_model;
+    P.alpha_source_floor = d.alpha_source_floor;
+    double *di0 = nullptr, *di1 = nullptr, *do0 = nullptr, *do1 = nullptr;
+    int* derr = nullptr;
+    auto cleanup = [&] {
+        hipFree(di0);
+        hipFree(di1);
+        hipFree(do0);
+        hipFree(do1);
+        hipFree(derr);
+    };
+    auto alloc = [&](double** p, size_t n) { return n ? hipMalloc(p, n * sizeof(double)) : hipSuccess; };
+    if (alloc(&di0, n_in0) != hipSuccess || alloc(&di1, n_in1) != hipSuccess || alloc(&do0, n_out0) != hipSuccess ||
+        alloc(&do1, n_out1) != hipSuccess || hipMalloc(&derr, sizeof(int)) != hipSuccess) {
+        cleanup();
+        return fail(nullptr, "dimhip_kat: device allocation failed");
+    }
+    hipMemset(derr, 0, sizeof(int));
+    hipMemcpy(di0, in0, n_in0 * sizeof(double), hipMemcpyHostToDevice);
+    if (n_in1) hipMemcpy(di1, in1, n_in1 * sizeof(double), hipMemcpyHostToDevice);
+    int rc = 0;
+    if (which == 5) {
+        kat_ieee_kernel<<<(count + 127) / 128, 128>>>(count, di0, do0, do1);
+    } else {
+        const Ops* ops = select_ops(D, M);
+        rc = ops ? ops->kat(P, which, dir, count, di0, di1, do0, do1, derr, nullptr) : 1;
+    }
+    hipError_t e = hipDeviceSynchronize();
+    int herr = 0;
+    if (e == hipSuccess) {
+        hipMemcpy(out0, do0, n_out0 * sizeof(double), hipMemcpyDeviceToHost);
+        if (n_out1) hipMemcpy(out1, do1, n_out1 * sizeof(double), hipMemcpyDeviceToHost);
+        hipMemcpy(&herr, derr, sizeof(int), hipMemcpyDeviceToHost);
+    }
+    cleanup();
+    if (e != hipSuccess) return fail(nullptr, std::string("HIP error: ") + hipGetErrorString(e));
+    if (rc) return fail(nullptr, "dimhip_kat: launch failed");
+    if (herr) return fail(nullptr, "dim::hllc_flux_normal: invalid left density (non-finite)");
+    return 0;
+}
+
+}  // extern "C"

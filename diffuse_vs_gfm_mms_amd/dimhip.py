@@ -1,0 +1,243 @@
+"""Python host binding of the gfx950 DIM library (include/dimhip.h) via ctypes.
+
+:class:`Solver` mirrors the reference's stepping interface
+``dim::advance_one_step(U, N, dx, params, cfg, cfl, dt_max)``
+(src/dim/solver/advance_step.hpp:20-149) over a device-resident state, plus
+the multi-step device loop of ``run_dim_case`` (src/app/solver/run_dim.hpp:88-111).
+Errors raise ``RuntimeError`` with the library's message, which keeps the
+reference's ``std::runtime_error`` wording where the reference has one.
+
+There is no CPU fallback: if ``_lib/libdimhip.so`` is missing, loading fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libdimhip.so")
+
+EOS = {"ideal_gas": 0, "stiffened_gas": 1}
+BC = {"transmissive": 0, "zero_gradient": 0, "outflow": 0, "reflective": 1,
+      "nonreflective": 2, "non_reflective": 2, "reflection": 3}
+LAMBDA = {"kapila": 0, "": 0, "allaire": 1, "equal_velocity": 2}
+TIME_UPDATE = {"split": 0, "unsplit": 1}
+PRECISION = {"fp64": 0, "fp32_mixed": 1}
+
+KAT_CONS_TO_PRIM, KAT_PRIM_TO_CONS, KAT_REPAIR, KAT_HLLC, KAT_ALPHA_RHS, KAT_IEEE = range(6)
+
+
+class Desc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int),
+        ("N", C.c_int * 3),
+        ("dx", C.c_double * 3),
+        ("nmat", C.c_int),
+        ("eos_kind", C.c_int * 4),
+        ("gamma", C.c_double * 4),
+        ("p_inf", C.c_double * 4),
+        ("bc_lo", C.c_int * 3),
+        ("bc_hi", C.c_int * 3),
+        ("lambda_model", C.c_int),
+        ("alpha_source_floor", C.c_double),
+        ("time_update", C.c_int),
+        ("precision", C.c_int),
+        ("device", C.c_int),
+    ]
+
+
+class Timings(C.Structure):
+    _fields_ = [("cfl_seconds", C.c_double), ("sweep_seconds", C.c_double),
+                ("boundary_seconds", C.c_double)]
+
+
+_lib = None
+
+
+def load_library():
+    """Load the in-tree HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    vp, dp, ip, llp = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_longlong)
+    sig = {
+        "dimhip_version": (C.c_int, []),
+        "dimhip_create": (C.c_int, [C.POINTER(Desc), C.POINTER(vp)]),
+        "dimhip_destroy": (C.c_int, [vp]),
+        "dimhip_upload": (C.c_int, [vp, dp, C.c_size_t]),
+        "dimhip_download": (C.c_int, [vp, dp, C.c_size_t]),
+        "dimhip_step": (C.c_int, [vp, C.c_double, C.c_double, dp, C.POINTER(Timings)]),
+        "dimhip_advance": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_int, dp, llp]),
+        "dimhip_compute_dt": (C.c_int, [vp, C.c_double, C.c_double, dp]),
+        "dimhip_sweep": (C.c_int, [vp, C.c_int, C.c_double]),
+        "dimhip_apply_bc": (C.c_int, [vp]),
+        "dimhip_conservation": (C.c_int, [vp, dp]),
+        "dimhip_synchronize": (C.c_int, [vp]),
+        "dimhip_set_timing": (C.c_int, [vp, C.c_int]),
+        "dimhip_kernel_times": (C.c_int, [vp, dp, llp]),
+        "dimhip_kat": (C.c_int, [C.POINTER(Desc), C.c_int, C.c_int, C.c_int, dp, dp, dp, dp]),
+        "dimhip_last_error": (C.c_char_p, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return ["dimhip_version", "dimhip_create", "dimhip_destroy", "dimhip_upload", "dimhip_download",
+            "dimhip_step", "dimhip_advance", "dimhip_compute_dt", "dimhip_sweep", "dimhip_apply_bc",
+            "dimhip_conservation", "dimhip_synchronize", "dimhip_set_timing", "dimhip_kernel_times",
+            "dimhip_kat", "dimhip_last_error"]
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_desc(dim, N, dx, materials, bc_lo, bc_hi, lambda_model="kapila", alpha_source_floor=0.0,
+              time_update="split", precision="fp64", device=0):
+    d = Desc()
+    d.dim = dim
+    for a in range(3):
+        d.N[a] = int(N[a]) if a < dim else 1
+        d.dx[a] = float(dx[a]) if a < dim else 1.0
+        d.bc_lo[a] = BC[bc_lo[a]] if a < dim else 0
+        d.bc_hi[a] = BC[bc_hi[a]] if a < dim else 0
+    d.nmat = len(materials)
+    for k, (kind, g, pinf) in enumerate(materials):
+        d.eos_kind[k] = EOS[kind]
+        d.gamma[k] = g
+        d.p_inf[k] = pinf
+    d.lambda_model = LAMBDA[lambda_model]
+    d.alpha_source_floor = alpha_source_floor
+    d.time_update = TIME_UPDATE[time_update]
+    d.precision = PRECISION[precision]
+    d.device = device
+    return d
+
+
+class Solver:
+    """Device-resident DIM state with the reference's stepping semantics."""
+
+    def __init__(self, dim, N, dx, materials, bc_lo, bc_hi, lambda_model="kapila",
+                 alpha_source_floor=0.0, time_update="split", precision="fp64", device=0):
+        self.lib = load_library()
+        self.dim, self.N = dim, tuple(int(n) for n in N)
+        self.nmat = len(materials)
+        self.nv = 2 * self.nmat + dim
+        self.ncells = int(np.prod(self.N))
+        self.desc = make_desc(dim, N, dx, materials, bc_lo, bc_hi, lambda_model, alpha_source_floor,
+                              time_update, precision, device)
+        h = C.c_void_p()
+        self._check(self.lib.dimhip_create(C.byref(self.desc), C.byref(h)), None)
+        self.h = h
+
+    @classmethod
+    def from_case(cls, case, precision="fp64", time_update="split", **kw):
+        s = cls(case.dim, case.N, case.dx, case.materials, case.bc_lo, case.bc_hi,
+                case.lambda_model, case.alpha_source_floor, time_update, precision, **kw)
+        s.upload(case.U)
+        return s
+
+    def _check(self, rc, h="self"):
+        if rc != 0:
+            handle = self.h if h == "self" else None
+            raise RuntimeError(self.lib.dimhip_last_error(handle).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dimhip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.size != self.ncells * self.nv:
+            raise RuntimeError("dim::advance_one_step: grid size mismatch")
+        self._check(self.lib.dimhip_upload(self.h, _dp(U), self.ncells))
+
+    def download(self):
+        out = np.empty((self.ncells, self.nv))
+        self._check(self.lib.dimhip_download(self.h, _dp(out), self.ncells))
+        return out
+
+    def step(self, cfl, dt_max):
+        """One dim::advance_one_step; returns dt."""
+        dt = C.c_double(0.0)
+        self._check(self.lib.dimhip_step(self.h, cfl, dt_max, C.byref(dt), None))
+        return dt.value
+
+    def advance(self, nsteps, cfl, t_end, time=0.0, use_graph=True):
+        """Up to nsteps loop iterations of run_dim_case with dt_max = t_end - time."""
+        t = C.c_double(time)
+        n = C.c_longlong(0)
+        self._check(self.lib.dimhip_advance(self.h, nsteps, cfl, t_end, int(use_graph),
+                                            C.byref(t), C.byref(n)))
+        return t.value, n.value
+
+    def compute_dt(self, cfl, dt_max):
+        dt = C.c_double(0.0)
+        self._check(self.lib.dimhip_compute_dt(self.h, cfl, dt_max, C.byref(dt)))
+        return dt.value
+
+    def sweep(self, d, dt):
+        self._check(self.lib.dimhip_sweep(self.h, d, dt))
+
+    def apply_bc(self):
+        self._check(self.lib.dimhip_apply_bc(self.h))
+
+    def conservation(self):
+        out = np.zeros(self.nmat + 1 + self.dim + 1)
+        self._check(self.lib.dimhip_conservation(self.h, _dp(out)))
+        return out
+
+    def synchronize(self):
+        self._check(self.lib.dimhip_synchronize(self.h))
+
+    def set_timing(self, on):
+        self._check(self.lib.dimhip_set_timing(self.h, int(on)))
+
+    def kernel_times(self):
+        """Mean device ms per launch of (sweep, boundary, cfl) kernels and launch counts."""
+        ms = np.zeros(3)
+        cnt = (C.c_longlong * 3)()
+        self._check(self.lib.dimhip_kernel_times(self.h, _dp(ms), cnt))
+        return ms, list(cnt)
+
+
+def kat(which, dim, materials, count, in0, in1=None, d=0, lambda_model="kapila",
+        alpha_source_floor=0.0):
+    """Run one device function of the path on `count` packed items (see include/dimhip.h)."""
+    lib = load_library()
+    desc = make_desc(dim, [3] * dim, [1.0] * dim, materials, ["transmissive"] * dim,
+                     ["transmissive"] * dim, lambda_model, alpha_source_floor)
+    nmat = len(materials)
+    nv, np_, nf = 2 * nmat + dim, 2 * nmat + dim + 1, nmat + dim + 1
+    in0 = np.ascontiguousarray(in0, dtype=np.float64)
+    in1 = np.ascontiguousarray(in1 if in1 is not None else np.zeros(1), dtype=np.float64)
+    size0 = {0: np_, 1: nv, 2: nv, 3: nf, 4: nmat, 5: 1}[which] * count
+    out0 = np.zeros(max(size0, 1))
+    out1 = np.zeros(max(count, 1))
+    rc = lib.dimhip_kat(C.byref(desc), which, d, count, _dp(in0), _dp(in1), _dp(out0), _dp(out1))
+    if rc != 0:
+        raise RuntimeError(lib.dimhip_last_error(None).decode())
+    return out0[:size0], out1[:count]

@@ -1,0 +1,156 @@
+/*
+ * dimhip.h — C ABI of the MI355X (gfx950) DIM library.
+ *
+ * The library replaces the reference's Allaire five-equation diffuse-interface
+ * update behind the boundary the reference's time loop calls:
+ *
+ *   dim::advance_one_step<DIM>(U, N, dx, params, cfg, cfl, dt_max) -> StepResult
+ *       src/dim/solver/advance_step.hpp:20-149, called from
+ *       src/app/solver/run_dim.hpp:94-102
+ *
+ * and the functions below it (compute_dt_cfl, sweep_direction_dispatch,
+ * apply_boundary_conditions).  State lives on the GPU between calls; the host
+ * exchanges it in the reference's packed per-cell order
+ *
+ *   partial_mass[nmat], mom[dim], E, alpha[nmat-1]   (nv = 2*nmat + dim doubles)
+ *
+ * (the order of src/app/dim_mpi_main.cpp:255-273), cells x-fastest
+ * (src/dim/solver/advance/geometry.hpp:25-55).
+ *
+ * Every entry point returns 0 on success and a nonzero status on failure;
+ * dimhip_last_error() returns the message, worded like the reference's
+ * std::runtime_error text where the reference has one.  A context is used by
+ * one host thread at a time and owns one HIP stream.
+ */
+#ifndef DIMHIP_H
+#define DIMHIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DIMHIP_VERSION 1
+#define DIMHIP_MAX_MAT 4
+
+/* EoS kinds on the path (src/euler/eos_params.hpp:6-13) */
+enum { DIMHIP_EOS_IDEAL = 0, DIMHIP_EOS_STIFFENED = 1 };
+
+/* Boundary names (config_loader.hpp:369-377; boundary.hpp:108-318).
+ * transmissive / zero_gradient / outflow map to TRANSMISSIVE; the literal
+ * "reflective" flips the normal momentum; "reflection" is accepted by the
+ * reference but copies without a flip (boundary.hpp:219-221). */
+enum {
+    DIMHIP_BC_TRANSMISSIVE = 0,
+    DIMHIP_BC_REFLECTIVE = 1,
+    DIMHIP_BC_NONREFLECTIVE = 2,
+    DIMHIP_BC_REFLECTION = 3
+};
+
+/* cfg.dim_lambda_model (src/io/config.hpp:121-124) */
+enum { DIMHIP_LAMBDA_KAPILA = 0, DIMHIP_LAMBDA_ALLAIRE = 1, DIMHIP_LAMBDA_EQUAL_VELOCITY = 2 };
+
+/* cfg.time_update (src/io/config.hpp:106) */
+enum { DIMHIP_SPLIT = 0, DIMHIP_UNSPLIT = 1 };
+
+/* Arithmetic form.  FP64: the reference's arithmetic, reproduced bit for bit.
+ * FP32_MIXED: FP32 state, reconstruction, fluxes and update; FP64 CFL
+ * reduction, time, conservation sums and interface-cell pressure recovery. */
+enum { DIMHIP_FP64 = 0, DIMHIP_FP32_MIXED = 1 };
+
+typedef struct {
+    int dim;                        /* 1..3 */
+    int N[3];                       /* cells per axis (unused axes: 1) */
+    double dx[3];                   /* cell size per axis */
+    int nmat;                       /* 1..4 */
+    int eos_kind[DIMHIP_MAX_MAT];   /* DIMHIP_EOS_* */
+    double gamma[DIMHIP_MAX_MAT];
+    double p_inf[DIMHIP_MAX_MAT];
+    int bc_lo[3];                   /* DIMHIP_BC_* */
+    int bc_hi[3];
+    int lambda_model;               /* DIMHIP_LAMBDA_* */
+    double alpha_source_floor;      /* cfg.dim_alpha_source_floor */
+    int time_update;                /* DIMHIP_SPLIT / DIMHIP_UNSPLIT */
+    int precision;                  /* DIMHIP_FP64 / DIMHIP_FP32_MIXED */
+    int device;                     /* HIP device ordinal */
+} dimhip_desc;
+
+/* Phase timings of one step, seconds (src/core/phase_timings.hpp:4-21). */
+typedef struct {
+    double cfl_seconds;
+    double sweep_seconds;
+    double boundary_seconds;
+} dimhip_timings;
+
+typedef struct dimhip_ctx dimhip_ctx;
+
+int dimhip_version(void);
+
+/* Create a context: validates the descriptor (dim::EOSParams::validate,
+ * src/dim/eos_params.hpp:17-29) and allocates the device state. */
+int dimhip_create(const dimhip_desc* desc, dimhip_ctx** out);
+int dimhip_destroy(dimhip_ctx* ctx);
+
+/* Host <-> device state in the packed per-cell order; ncells must equal
+ * N[0]*N[1]*N[2] (advance_step.hpp:43-46). */
+int dimhip_upload(dimhip_ctx* ctx, const double* packed, size_t ncells);
+int dimhip_download(dimhip_ctx* ctx, double* packed, size_t ncells);
+
+/* One dim::advance_one_step (advance_step.hpp:29-149): CFL dt (cfl.hpp:15-119),
+ * directional sweeps x->y->z (or unsplit), ring boundary conditions after each
+ * direction.  *dt_out receives the step's dt; timings may be NULL. */
+int dimhip_step(dimhip_ctx* ctx, double cfl, double dt_max, double* dt_out,
+                dimhip_timings* timings);
+
+/* Up to nsteps iterations of run_dim_case's loop body (run_dim.hpp:88-111)
+ * with dt_max = t_end - time, time kept on the device; steps stop once
+ * time >= t_end - 1e-14.  No host synchronisation between steps; with
+ * use_graph != 0 the step sequence is captured once in a hipGraph and
+ * replayed.  *time_inout is the start time on entry and the end time on
+ * return; *steps_done the number of steps taken. */
+int dimhip_advance(dimhip_ctx* ctx, int nsteps, double cfl, double t_end, int use_graph,
+                   double* time_inout, long long* steps_done);
+
+/* compute_dt_cfl / compute_dt_cfl_unsplit on the current state (cfl.hpp:15-119). */
+int dimhip_compute_dt(dimhip_ctx* ctx, double cfl, double dt_max, double* dt_out);
+
+/* sweep_direction_dispatch (sweep_core.hpp:151-202) on the current state with
+ * time step dt; the result replaces the state.  No boundary pass. */
+int dimhip_sweep(dimhip_ctx* ctx, int dir, double dt);
+
+/* apply_boundary_conditions (boundary.hpp:165-226) on the current state. */
+int dimhip_apply_bc(dimhip_ctx* ctx);
+
+/* FP64 conservation totals (conservation_report.hpp:188-227), fixed-order
+ * tree sums: out = [mass_k * V (nmat), mass * V, mom_d * V (dim), E * V]. */
+int dimhip_conservation(dimhip_ctx* ctx, double* out);
+
+/* Wait for the context's stream. */
+int dimhip_synchronize(dimhip_ctx* ctx);
+
+/* Mean device duration (ms) of each kernel class over the calls since the
+ * last reset, measured with HIP events on the context's stream:
+ * out[0] sweep kernels, out[1] boundary kernels, out[2] CFL kernels;
+ * counts[i] = launches measured.  Timing is enabled by dimhip_set_timing. */
+int dimhip_set_timing(dimhip_ctx* ctx, int enable);
+int dimhip_kernel_times(dimhip_ctx* ctx, double* mean_ms, long long* counts);
+
+/* Known-answer entry point for the device functions on the path.  `which`:
+ *   0 cons_to_prim   (primitives.hpp:111-162)  in0: count*nv cells   -> out0: count*(2nmat+dim+1)
+ *   1 prim_to_cons   (primitives.hpp:166-212)  in0: count*(2nmat+dim+1) -> out0: count*nv
+ *   2 repair_state   (primitives.hpp:326-457)  in0: count*nv -> out0: count*nv
+ *   3 hllc_flux_normal (hllc.hpp:62-195) along `dir`; in0/in1: left/right cells
+ *                    -> out0: count*(nmat+dim+1) fluxes, out1: count face velocities
+ *   4 alpha_rhs_coefficients (primitives.hpp:216-322) in0: primitives -> out0: count*nmat
+ *   5 IEEE check: out0[i] = in0[2i] / in0[2i+1], out1[i] = sqrt(in0[2i]) in FP64
+ * Host buffers; the call allocates and frees its own device memory. */
+int dimhip_kat(const dimhip_desc* desc, int which, int dir, int count, const double* in0,
+               const double* in1, double* out0, double* out1);
+
+const char* dimhip_last_error(const dimhip_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,81 @@
+"""Seeded state generators shared by the tests (inputs only)."""
+import numpy as np
+
+WATER = ("stiffened_gas", 4.4, 6.0e8)
+AIR = ("ideal_gas", 1.4, 0.0)
+HELIUM = ("ideal_gas", 1.648, 0.0)
+MATSETS = {"water_air": [WATER, AIR], "air_helium": [AIR, HELIUM],
+           "air": [AIR], "water_air_helium": [WATER, AIR, HELIUM]}
+
+
+def random_states(materials, dim, n, rng, extreme=True):
+    """Random packed cells over a wide physical range plus edge cases the
+    reference's guards handle: vanishing / negative volume fractions and
+    partial masses, and total energy below the internal-energy floor."""
+    nmat = len(materials)
+    nv = 2 * nmat + dim
+    a = rng.uniform(0.0, 1.0, size=(n, nmat))
+    choice = rng.integers(0, 10, size=n) if extreme else np.full(n, 9)
+    a[choice == 0, 0] = 1e-12
+    a[choice == 1, 0] = 1.0 - 1e-9
+    a[choice == 2, 0] = 1e-7
+    a[choice == 3, 0] = -1e-9
+    a[choice == 4, 0] = 1.0
+    a = a / a.sum(1, keepdims=True)
+    rho = 10.0 ** rng.uniform(-1, 3.3, size=(n, nmat))
+    u = rng.uniform(-400, 400, size=(n, dim))
+    p = 10.0 ** rng.uniform(3, 9.3, size=n)
+    m = a * rho
+    if nmat > 1:
+        m[choice == 5, 1] = 0.0
+    m[choice == 6, 0] = -1e-9
+    rt = m.sum(1)
+    e = np.zeros(n)
+    for k, (kind, g, pinf) in enumerate(materials):
+        e += a[:, k] * (p + (g * pinf if kind == "stiffened_gas" else 0.0)) / (g - 1.0)
+    E = e + 0.5 * rt * (u ** 2).sum(1)
+    sel = choice == 7
+    E[sel] *= rng.uniform(-0.5, 0.5, size=sel.sum())
+    U = np.zeros((n, nv))
+    U[:, :nmat] = m
+    U[:, nmat:nmat + dim] = rt[:, None] * u
+    U[:, nmat + dim] = E
+    U[:, nmat + dim + 1:] = a[:, :nmat - 1]
+    return U
+
+
+def random_line(materials, dim, n, rng):
+    """A line with a few constant pieces (discontinuities) and some noise."""
+    base = random_states(materials, dim, 4, rng, extreme=False)
+    cuts = np.sort(rng.integers(1, max(n - 1, 2), size=3))
+    piece = np.searchsorted(cuts, np.arange(n), side="right")
+    U = base[piece].copy()
+    U *= 1.0 + 1e-3 * rng.standard_normal(U.shape)
+    return U
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    return a.shape == b.shape and bool(np.all(a.view(np.int64) == b.view(np.int64)))
+
+
+def mismatch_report(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    bad = a.view(np.int64) != b.view(np.int64)
+    n = int(bad.sum())
+    if n == 0:
+        return "bit-identical"
+    denom = np.maximum(np.abs(b), 1e-300)
+    rel = np.max(np.abs(a - b)[bad] / denom[bad])
+    first = np.argwhere(bad)[0]
+    return f"{n} mismatching values, max rel {rel:.3e}, first at {tuple(first)}"
+
+
+def rel_linf(a, b):
+    """Relative L-infinity per field (column): max|a-b| / max|b|."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = np.maximum(np.max(np.abs(b), axis=0), 1e-300)
+    return np.max(np.abs(a - b), axis=0) / scale
