@@ -186,7 +186,6 @@ sweep64_kernel(const double* __restrict__ qin, double* __restrict__ qout, Grid g
     constexpr int NREC = 4 * M + 2 * D + 1;
     __shared__ double sh[NREC * S];
 
-    if (!ctl->active) return;
     const double dt = ctl->dt;
     const double dxs = g.dx[DIR];
 
@@ -226,6 +225,17 @@ sweep64_kernel(const double* __restrict__ qin, double* __restrict__ qout, Grid g
     const bool in = line_ok && i >= 0 && i < n;
     const long long cell = (long long)idx[0] + (long long)idx[1] * g.stride[1] + (long long)idx[2] * g.stride[2];
     const bool end_cell = (i == 0) || (i == n - 1);
+
+    if (!ctl->active) {
+        // a step past t_end: the state passes through unchanged, so the
+        // host-side double-buffer flip stays consistent
+        if (MODE == 0 && in && pos >= 3 && pos < L - 3) {
+            State<D, M> U;
+            load_state<D, M>(qin, g.pitch, cell, U);
+            store_state<D, M>(qout, g.pitch, cell, U);
+        }
+        return;
+    }
 
     // phase 1
     State<D, M> U;
