@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="helium_fp64", choices=sorted(WORKLOADS))
+    ap.add_argument("--fp32-steps", type=int, default=2000,
+                    help="steps of the FP32-mixed 4096x2048 companion measurement (0: skip)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
@@ -74,6 +76,34 @@ def cpu_baseline(case, nsteps):
     }
 
 
+def sweep_bytes(case, precision):
+    """Algorithmic bytes of one sweep launch: every stored field read and written once."""
+    if precision == "fp64":
+        per_cell = 8 * case.nv
+    else:  # FP64 partial masses + FP32 momentum, energy, volume fractions
+        per_cell = 8 * case.nmat + 4 * (case.nv - case.nmat)
+    return 2 * per_cell * case.ncells
+
+
+def measure(dimhip, workloads, grid, precision, steps, warmup):
+    case = workloads.helium_shock_bubble_2d(*grid)
+    t_end = 1.0  # far beyond the steps taken: dt is CFL-limited throughout
+    solver = dimhip.Solver.from_case(case, precision=precision)
+    t, n = solver.advance(warmup, case.cfl, t_end, 0.0, use_graph=False)
+    solver.synchronize()
+    solver.set_timing(True)
+    t0 = time.perf_counter()
+    t, n = solver.advance(steps, case.cfl, t_end, t, use_graph=False)
+    solver.synchronize()
+    wall = time.perf_counter() - t0
+    ms, counts = solver.kernel_times()
+    solver.set_timing(False)
+    solver.close()
+    if n != steps:
+        raise RuntimeError(f"took {n} steps, expected {steps}")
+    return {"case": case, "value": case.ncells * steps / wall, "wall": wall, "ms": ms, "counts": counts}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -82,28 +112,16 @@ def main():
         sys.exit(2)
     from diffuse_vs_gfm_mms_amd import dimhip, workloads
 
-    grid, precision, vbytes = WORKLOADS[args.workload]
-    case = workloads.helium_shock_bubble_2d(*grid)
-    t_end = 1.0  # far beyond the steps taken: dt is CFL-limited throughout
-
-    solver = dimhip.Solver.from_case(case, precision=precision)
-    t, n = solver.advance(args.warmup, case.cfl, t_end, 0.0, use_graph=False)
-    solver.synchronize()
-    solver.set_timing(True)
-    t0 = time.perf_counter()
-    t, n = solver.advance(args.steps, case.cfl, t_end, t, use_graph=False)
-    solver.synchronize()
-    wall = time.perf_counter() - t0
-    ms, counts = solver.kernel_times()
-    solver.set_timing(False)
-    if n != args.steps:
-        raise RuntimeError(f"took {n} steps, expected {args.steps}")
-    solver.close()
-
+    grid, precision, _ = WORKLOADS[args.workload]
+    main_run = measure(dimhip, workloads, grid, precision, args.steps, args.warmup)
+    case = main_run["case"]
     cells = case.ncells
-    value = cells * args.steps / wall
+    value = main_run["value"]
+    wall = main_run["wall"]
+    ms = main_run["ms"]
+    counts = main_run["counts"]
     sweep_ms = ms[0]
-    bytes_per_launch = 2 * case.nv * vbytes * cells
+    bytes_per_launch = sweep_bytes(case, precision)
     achieved = bytes_per_launch / (sweep_ms * 1e-3) / 1e9
     out = {
         "metric": "cell-updates/s, 2D DIM MUSCL-Hancock+HLLC, FP32-mixed vs FP64; % HBM peak",
@@ -139,6 +157,23 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if args.fp32_steps > 0 and precision == "fp64":
+        g32 = WORKLOADS["helium_fp32"][0]
+        r32 = measure(dimhip, workloads, g32, "fp32_mixed", args.fp32_steps, args.warmup)
+        b32 = sweep_bytes(r32["case"], "fp32_mixed")
+        a32 = b32 / (r32["ms"][0] * 1e-3) / 1e9
+        out["fp32_mixed"] = {
+            "workload": f"helium shock-bubble {g32[0]}x{g32[1]}, fp32_mixed, split x->y, CFL 0.8",
+            "value": r32["value"],
+            "unit": "cell-updates/s",
+            "steps": args.fp32_steps,
+            "ms_per_step": r32["wall"] / args.fp32_steps * 1e3,
+            "speedup_vs_fp64_cell_rate": r32["value"] / value,
+            "roofline": {"bound": "hbm", "achieved": a32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": a32 / HBM_PEAK_GBS, "traffic": traffic_for("helium_fp32"),
+                         "mean_ms": {"sweep": r32["ms"][0], "boundary": r32["ms"][1], "cfl": r32["ms"][2]},
+                         "bytes_per_launch": b32},
+        }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(workloads.helium_shock_bubble_2d(2048, 1024), args.cpu_steps)
     print(json.dumps(out))

@@ -14,7 +14,7 @@
 
 #include "../../include/dimhip.h"
 #include "kernels_fp64.hpp"
-#include "kernels_fp32.hpp"
+#include "policy_fp32.hpp"
 
 using namespace dimhip;
 
@@ -35,14 +35,12 @@ struct dimhip_ctx {
     dimhip_desc desc{};
     Grid grid{};
     EosParams eos{};
-    EosParamsF eosf{};
+    EosMixed eosm{};
     BcArgs bc{};
     int nv = 0;
+    size_t cell_bytes = 0;  // storage bytes per cell
     hipStream_t stream = nullptr;
-    double* q[2] = {nullptr, nullptr};
-    double* qacc = nullptr;
-    float* qf[2] = {nullptr, nullptr};
-    float* qfacc = nullptr;
+    void* q[2] = {nullptr, nullptr};
     int cur = 0;
     StepCtl* ctl = nullptr;
     StepCtl* ctl_host = nullptr;  // pinned
@@ -130,6 +128,7 @@ struct Ops {
     virtual void bc(dimhip_ctx* c, void* q, int end_step) const = 0;
     virtual void cfl(dimhip_ctx* c, const void* q, bool unsplit, double cfl) const = 0;
     virtual void totals(dimhip_ctx* c, const void* q) const = 0;
+    virtual void pack(dimhip_ctx* c, bool in, double* staging, void* q, long long first, long long count) const = 0;
     virtual int kat(const EosParams& P, int which, int dir, int count, const double* in0, const double* in1,
                     double* out0, double* out1, int* err, hipStream_t s) const = 0;
 };
@@ -156,38 +155,37 @@ int grid_blocks(long long work, int per)
 
 template<int D, int M>
 struct OpsImpl final : Ops {
-    template<int DIR, int MODE, bool FUSE>
-    static void launch_sweep64(dimhip_ctx* c, const double* in, double* out, const CflArgs& ca)
+    template<typename Pol, int DIR, int MODE, bool FUSE>
+    static void launch_sweep(dimhip_ctx* c, const void* in, void* out, const typename Pol::Eos& P, const CflArgs& ca)
     {
         const Grid& g = c->grid;
         if constexpr (DIR == 0) {
             constexpr int T = kSweepL0 - 6;
             dim3 blk(kSweepL0, kSweepW0, 1);
             dim3 grd((g.N[0] + T - 1) / T, (g.N[1] + kSweepW0 - 1) / kSweepW0, g.N[2]);
-            sweep64_kernel<D, M, 0, kSweepL0, kSweepW0, MODE, FUSE><<<grd, blk, 0, c->stream>>>(in, out, g, c->eos, c->ctl, ca);
+            sweep_kernel<Pol, D, M, 0, kSweepL0, kSweepW0, MODE, FUSE><<<grd, blk, 0, c->stream>>>(in, out, g, P, c->ctl, ca);
         } else {
             constexpr int T = kSweepL1 - 6;
             dim3 blk(kSweepW1, kSweepL1, 1);
             dim3 grd((g.N[0] + kSweepW1 - 1) / kSweepW1, (g.N[DIR] + T - 1) / T, DIR == 1 ? g.N[2] : g.N[1]);
-            sweep64_kernel<D, M, DIR, kSweepL1, kSweepW1, MODE, FUSE><<<grd, blk, 0, c->stream>>>(in, out, g, c->eos, c->ctl, ca);
+            sweep_kernel<Pol, D, M, DIR, kSweepL1, kSweepW1, MODE, FUSE><<<grd, blk, 0, c->stream>>>(in, out, g, P, c->ctl, ca);
         }
+    }
+
+    template<typename Pol, int DIR>
+    static void sweep_pol(dimhip_ctx* c, int mode, bool fuse, const void* in, void* out, const typename Pol::Eos& P)
+    {
+        const CflArgs ca = make_cfl_args(c, c->dt_ready_cfl, c->desc.time_update == DIMHIP_UNSPLIT);
+        if (mode == 0 && !fuse) launch_sweep<Pol, DIR, 0, false>(c, in, out, P, ca);
+        else if (mode == 0 && fuse) launch_sweep<Pol, DIR, 0, true>(c, in, out, P, ca);
+        else launch_sweep<Pol, DIR, 1, false>(c, in, out, P, ca);
     }
 
     template<int DIR>
     static void sweep_dir(dimhip_ctx* c, int mode, bool fuse, const void* in, void* out)
     {
-        const CflArgs ca = make_cfl_args(c, c->dt_ready_cfl, c->desc.time_update == DIMHIP_UNSPLIT);
-        if (c->desc.precision == DIMHIP_FP64) {
-            const double* qi = (const double*)in;
-            double* qo = (double*)out;
-            if (mode == 0 && !fuse) launch_sweep64<DIR, 0, false>(c, qi, qo, ca);
-            else if (mode == 0 && fuse) launch_sweep64<DIR, 0, true>(c, qi, qo, ca);
-            else if (mode == 1 && !fuse) launch_sweep64<DIR, 1, false>(c, qi, qo, ca);
-            else launch_sweep64<DIR, 1, true>(c, qi, qo, ca);
-        } else {
-            launch_sweep32<D, M, DIR>(c->stream, c->grid, c->eosf, c->eos, c->ctl, ca, mode, fuse,
-                                     (const float*)in, (float*)out);
-        }
+        if (c->desc.precision == DIMHIP_FP64) sweep_pol<Fp64, DIR>(c, mode, fuse, in, out, c->eos);
+        else sweep_pol<Fp32Mixed, DIR>(c, mode, fuse, in, out, c->eosm);
     }
 
     void sweep(dimhip_ctx* c, int dir, int mode, bool fuse, const void* in, void* out) const override
@@ -204,9 +202,9 @@ struct OpsImpl final : Ops {
         const long long total = c->bc.face_start[2 * D];
         const int blocks = grid_blocks(total, 256);
         if (c->desc.precision == DIMHIP_FP64)
-            bc64_kernel<D, M><<<blocks, 256, 0, c->stream>>>((double*)q, c->grid, c->eos, c->bc, c->ctl, end_step);
+            bc_kernel<Fp64, D, M><<<blocks, 256, 0, c->stream>>>(q, c->grid, c->eos, c->bc, c->ctl, end_step);
         else
-            bc32_kernel<D, M><<<blocks, 256, 0, c->stream>>>((float*)q, c->grid, c->eosf, c->eos, c->bc, c->ctl, end_step);
+            bc_kernel<Fp32Mixed, D, M><<<blocks, 256, 0, c->stream>>>(q, c->grid, c->eosm, c->bc, c->ctl, end_step);
     }
 
     void cfl(dimhip_ctx* c, const void* q, bool unsplit, double cfl) const override
@@ -214,9 +212,9 @@ struct OpsImpl final : Ops {
         const CflArgs ca = make_cfl_args(c, cfl, unsplit);
         const int blocks = grid_blocks(c->grid.ncells, 256 * 4);
         if (c->desc.precision == DIMHIP_FP64)
-            cfl64_kernel<D, M><<<blocks, 256, 0, c->stream>>>((const double*)q, c->grid, c->eos, c->ctl, ca);
+            cfl_kernel<Fp64, D, M><<<blocks, 256, 0, c->stream>>>(q, c->grid, c->eos, c->ctl, ca);
         else
-            cfl32_kernel<D, M><<<blocks, 256, 0, c->stream>>>((const float*)q, c->grid, c->eos, c->ctl, ca);
+            cfl_kernel<Fp32Mixed, D, M><<<blocks, 256, 0, c->stream>>>(q, c->grid, c->eosm, c->ctl, ca);
     }
 
     void totals(dimhip_ctx* c, const void* q) const override
@@ -225,10 +223,22 @@ struct OpsImpl final : Ops {
         for (int d = 0; d < D; ++d) volume *= c->grid.dx[d];
         const long long chunk = (c->grid.ncells + c->partial_blocks - 1) / c->partial_blocks;
         if (c->desc.precision == DIMHIP_FP64)
-            cons_totals_kernel<D, M><<<c->partial_blocks, 256, 0, c->stream>>>((const double*)q, c->grid, volume, chunk, c->partial);
+            totals_kernel<Fp64, D, M><<<c->partial_blocks, 256, 0, c->stream>>>(q, c->grid, c->eos, volume, chunk, c->partial);
         else
-            cons_totals32_kernel<D, M><<<c->partial_blocks, 256, 0, c->stream>>>((const float*)q, c->grid, volume, chunk, c->partial);
+            totals_kernel<Fp32Mixed, D, M><<<c->partial_blocks, 256, 0, c->stream>>>(q, c->grid, c->eosm, volume, chunk, c->partial);
         sum_partials_kernel<<<1, 64, 0, c->stream>>>(c->partial, c->partial_blocks, M + 1 + D + 1, c->totals);
+    }
+
+    void pack(dimhip_ctx* c, bool in, double* staging, void* q, long long first, long long count) const override
+    {
+        const int blocks = grid_blocks(count, 256);
+        if (c->desc.precision == DIMHIP_FP64) {
+            if (in) pack_in_kernel<Fp64, D, M><<<blocks, 256, 0, c->stream>>>(c->eos, staging, q, first, count, c->grid.pitch);
+            else pack_out_kernel<Fp64, D, M><<<blocks, 256, 0, c->stream>>>(c->eos, q, staging, first, count, c->grid.pitch);
+        } else {
+            if (in) pack_in_kernel<Fp32Mixed, D, M><<<blocks, 256, 0, c->stream>>>(c->eosm, staging, q, first, count, c->grid.pitch);
+            else pack_out_kernel<Fp32Mixed, D, M><<<blocks, 256, 0, c->stream>>>(c->eosm, q, staging, first, count, c->grid.pitch);
+        }
     }
 
     int kat(const EosParams& P, int which, int dir, int count, const double* in0, const double* in1, double* out0,
@@ -292,7 +302,7 @@ int validate_desc(const dimhip_desc* d, std::string& msg)
     }
     if (d->lambda_model < 0 || d->lambda_model > 2) return msg = "dim::alpha_rhs_coefficients: unknown lambda model", 1;
     if (d->time_update != DIMHIP_SPLIT && d->time_update != DIMHIP_UNSPLIT) return msg = "dimhip: unknown time_update", 1;
-    if (d->precision != DIMHIP_FP64) return msg = "dimhip: unknown precision", 1;
+    if (d->precision != DIMHIP_FP64 && d->precision != DIMHIP_FP32_MIXED) return msg = "dimhip: unknown precision", 1;
     return 0;
 }
 
@@ -329,26 +339,13 @@ __global__ void copy_kernel(const double* __restrict__ a, double* __restrict__ b
         b[t] = a[t];
 }
 
-__global__ void copy32_kernel(const float* __restrict__ a, float* __restrict__ b, long long n)
-{
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
-        b[t] = a[t];
-}
-
-void* state_ptr(dimhip_ctx* c, int which)
-{
-    return c->desc.precision == DIMHIP_FP64 ? (void*)c->q[which] : (void*)c->qf[which];
-}
-
-void* acc_ptr(dimhip_ctx* c) { return c->desc.precision == DIMHIP_FP64 ? (void*)c->qacc : (void*)c->qfacc; }
+void* state_ptr(dimhip_ctx* c, int which) { return c->q[which]; }
 
 void copy_state(dimhip_ctx* c, const void* a, void* b)
 {
-    const long long n = c->grid.pitch * c->nv;
-    if (c->desc.precision == DIMHIP_FP64)
-        copy_kernel<<<grid_blocks(n, 1024), 256, 0, c->stream>>>((const double*)a, (double*)b, n);
-    else
-        copy32_kernel<<<grid_blocks(n, 1024), 256, 0, c->stream>>>((const float*)a, (float*)b, n);
+    // storage sizes are whole doubles: pitch is a multiple of 64
+    const long long n = (long long)(c->grid.pitch * c->cell_bytes / sizeof(double));
+    copy_kernel<<<grid_blocks(n, 1024), 256, 0, c->stream>>>((const double*)a, (double*)b, n);
 }
 
 bool fusable(const dimhip_ctx* c)
@@ -468,17 +465,20 @@ int dimhip_create(const dimhip_desc* desc, dimhip_ctx** out)
         c->eos.pinf[k] = on ? desc->p_inf[k] : 0.0;
         c->eos.gm1[k] = c->eos.gamma[k] - 1.0;
         c->eos.gpinf[k] = c->eos.gamma[k] * c->eos.pinf[k];
-        c->eosf.kind[k] = c->eos.kind[k];
-        c->eosf.gamma[k] = (float)c->eos.gamma[k];
-        c->eosf.pinf[k] = (float)c->eos.pinf[k];
-        c->eosf.gm1[k] = (float)c->eos.gm1[k];
-        c->eosf.gpinf[k] = (float)c->eos.gpinf[k];
-        c->eosf.inv_gm1[k] = (float)(1.0 / c->eos.gm1[k]);
+        c->eosm.kind[k] = c->eos.kind[k];
+        c->eosm.gamma[k] = (float)c->eos.gamma[k];
+        c->eosm.pinf[k] = (float)(c->eos.kind[k] == EOS_STIFFENED ? c->eos.pinf[k] : 0.0);
+        c->eosm.gpinf[k] = (float)(c->eos.kind[k] == EOS_STIFFENED ? c->eos.gpinf[k] : 0.0);
+        c->eosm.inv_gm1_d[k] = 1.0 / c->eos.gm1[k];
+        c->eosm.b_d[k] = (c->eos.kind[k] == EOS_STIFFENED ? c->eos.gpinf[k] : 0.0) / c->eos.gm1[k];
+        c->eosm.inv_gm1[k] = (float)c->eosm.inv_gm1_d[k];
+        c->eosm.b[k] = (float)c->eosm.b_d[k];
+        c->eosm.bmin[k] = (float)(1e-10 / c->eos.gm1[k]);
     }
     c->eos.lambda_model = desc->lambda_model;
     c->eos.alpha_source_floor = desc->alpha_source_floor;
-    c->eosf.lambda_model = desc->lambda_model;
-    c->eosf.alpha_source_floor = (float)desc->alpha_source_floor;
+    c->eosm.lambda_model = desc->lambda_model;
+    c->eosm.alpha_source_floor = (float)desc->alpha_source_floor;
     // ring cells per face (d, side), owned by the face of their lowest boundary axis
     long long acc = 0;
     for (int f = 0; f < 6; ++f) {
@@ -500,16 +500,14 @@ int dimhip_create(const dimhip_desc* desc, dimhip_ctx** out)
         c->bc.bc_hi[d] = d < desc->dim ? desc->bc_hi[d] : 0;
     }
     HIP_OK(nullptr, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    const size_t bytes = (size_t)g.pitch * c->nv * (desc->precision == DIMHIP_FP64 ? sizeof(double) : sizeof(float));
-    if (desc->precision == DIMHIP_FP64) {
-        HIP_OK(nullptr, hipMalloc(&c->q[0], bytes));
-        HIP_OK(nullptr, hipMalloc(&c->q[1], bytes));
-    } else {
-        HIP_OK(nullptr, hipMalloc(&c->qf[0], bytes));
-        HIP_OK(nullptr, hipMalloc(&c->qf[1], bytes));
-    }
-    HIP_OK(nullptr, hipMemsetAsync(state_ptr(c.get(), 0), 0, bytes, c->stream));
-    HIP_OK(nullptr, hipMemsetAsync(state_ptr(c.get(), 1), 0, bytes, c->stream));
+    c->cell_bytes = desc->precision == DIMHIP_FP64
+                        ? sizeof(double) * c->nv
+                        : sizeof(double) * desc->nmat + sizeof(float) * (desc->dim + desc->nmat);
+    const size_t bytes = (size_t)g.pitch * c->cell_bytes;
+    HIP_OK(nullptr, hipMalloc(&c->q[0], bytes));
+    HIP_OK(nullptr, hipMalloc(&c->q[1], bytes));
+    HIP_OK(nullptr, hipMemsetAsync(c->q[0], 0, bytes, c->stream));
+    HIP_OK(nullptr, hipMemsetAsync(c->q[1], 0, bytes, c->stream));
     HIP_OK(nullptr, hipMalloc(&c->ctl, sizeof(StepCtl)));
     HIP_OK(nullptr, hipHostMalloc(&c->ctl_host, sizeof(StepCtl), hipHostMallocDefault));
     c->partial_blocks = (int)std::min<long long>(1024, std::max<long long>(1, g.ncells / 4096));
@@ -531,10 +529,6 @@ int dimhip_destroy(dimhip_ctx* c)
     for (auto e : c->ev_pool) hipEventDestroy(e);
     hipFree(c->q[0]);
     hipFree(c->q[1]);
-    hipFree(c->qacc);
-    hipFree(c->qf[0]);
-    hipFree(c->qf[1]);
-    hipFree(c->qfacc);
     hipFree(c->ctl);
     hipHostFree(c->ctl_host);
     hipFree(c->staging);
@@ -561,10 +555,7 @@ int dimhip_upload(dimhip_ctx* c, const double* packed, size_t ncells)
         const long long cnt = std::min(chunk, c->grid.ncells - first);
         HIP_OK(c, hipMemcpyAsync(c->staging, packed + first * c->nv, sizeof(double) * (size_t)(cnt * c->nv),
                                  hipMemcpyHostToDevice, c->stream));
-        if (c->desc.precision == DIMHIP_FP64)
-            aos_to_soa_kernel<<<grid_blocks(cnt * c->nv, 1024), 256, 0, c->stream>>>(c->staging, c->q[c->cur], first, cnt, c->nv, c->grid.pitch);
-        else
-            aos_to_soa32_kernel<<<grid_blocks(cnt * c->nv, 1024), 256, 0, c->stream>>>(c->staging, c->qf[c->cur], first, cnt, c->nv, c->grid.pitch);
+        c->ops->pack(c, true, c->staging, c->q[c->cur], first, cnt);
         HIP_OK(c, hipGetLastError());
         HIP_OK(c, hipStreamSynchronize(c->stream));
     }
@@ -586,10 +577,7 @@ int dimhip_download(dimhip_ctx* c, double* packed, size_t ncells)
     }
     for (long long first = 0; first < c->grid.ncells; first += chunk) {
         const long long cnt = std::min(chunk, c->grid.ncells - first);
-        if (c->desc.precision == DIMHIP_FP64)
-            soa_to_aos_kernel<<<grid_blocks(cnt * c->nv, 1024), 256, 0, c->stream>>>(c->q[c->cur], c->staging, first, cnt, c->nv, c->grid.pitch);
-        else
-            soa32_to_aos_kernel<<<grid_blocks(cnt * c->nv, 1024), 256, 0, c->stream>>>(c->qf[c->cur], c->staging, first, cnt, c->nv, c->grid.pitch);
+        c->ops->pack(c, false, c->staging, c->q[c->cur], first, cnt);
         HIP_OK(c, hipGetLastError());
         HIP_OK(c, hipMemcpyAsync(packed + first * c->nv, c->staging, sizeof(double) * (size_t)(cnt * c->nv),
                                  hipMemcpyDeviceToHost, c->stream));

@@ -1,0 +1,895 @@
+// policy_fp32.hpp — the mixed-precision policy of the generic DIM kernels.
+//
+// Same scheme as the reference (two-stage MUSCL-Hancock, HLLC, Kapila alpha
+// source, repair, ring BCs), different arithmetic:
+//  * reconstruction, fluxes, half step and the update of momentum, energy and
+//    volume fraction run in FP32 and are stored in FP32;
+//  * partial masses are stored in FP64 and updated in FP64 from the FP32
+//    face fluxes, so mass telescopes to FP64 rounding;
+//  * the mixture pressure is the closed-form root of
+//    rho e = sum_k alpha_k (p + gamma_k p_inf_k)/(gamma_k - 1) (the equation the
+//    reference bisects, src/dim/eos.hpp:77-143), evaluated in FP64 in
+//    interface cells (1e-6 < alpha_k < 1-1e-6) where stiffened-gas
+//    cancellation bites, in FP32 elsewhere;
+//  * the CFL candidate cfl*dx/(|u|+c) and its reduction are FP64.
+// Divisions use the hardware reciprocal (v_rcp_f32, 1 ulp) and square roots
+// v_sqrt_f32 / v_rsq_f32.
+#pragma once
+
+#include "policy_fp64.hpp"
+
+namespace dimhip {
+
+struct EosMixed {
+    int kind[kMaxMat];
+    float gamma[kMaxMat];
+    float pinf[kMaxMat];
+    float gpinf[kMaxMat];     // gamma * p_inf
+    float inv_gm1[kMaxMat];   // 1/(gamma-1)
+    float b[kMaxMat];         // gamma p_inf/(gamma-1)
+    float bmin[kMaxMat];      // 1e-10/(gamma-1): shifted energy floor per unit volume fraction
+    double inv_gm1_d[kMaxMat];
+    double b_d[kMaxMat];
+    int lambda_model;
+    float alpha_source_floor;
+};
+
+namespace f32 {
+
+__device__ __forceinline__ float fmaxs(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ float fmins(float a, float b) { return (b < a) ? b : a; }
+template<typename T>
+__device__ __forceinline__ bool ffinite(T x) { return __builtin_isfinite(x); }
+__device__ __forceinline__ float ffinite_or(float x, float fb = 0.f) { return ffinite(x) ? x : fb; }
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float safe_denom_f(float d, float tol = 1e-10f)
+{
+    if (!ffinite(d) || __builtin_fabsf(d) < tol) return (d < 0.f) ? -tol : tol;
+    return d;
+}
+__device__ __forceinline__ float safe_div_f(float n, float d, float tol = 1e-10f)
+{
+    return ffinite_or(n) * rcp(safe_denom_f(d, tol));
+}
+
+template<int D, int M>
+struct St {
+    float m[M];
+    float mom[D];
+    float E;
+    float a[M > 1 ? M - 1 : 1];
+};
+
+template<int D, int M>
+struct Pr {
+    float rho[M];
+    float alpha[M];
+    float vel[D];
+    float p;
+};
+
+template<int D, int M>
+struct Fx {
+    float m[M];
+    float mom[D];
+    float E;
+};
+
+template<int D, int M>
+struct Cell {
+    double m[M];
+    float mom[D];
+    float E;
+    float a[M > 1 ? M - 1 : 1];
+};
+
+template<int M>
+__device__ __forceinline__ void sanitise(float* alpha, float floor)
+{
+    float sum = 0.f;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        alpha[k] = fmaxs(ffinite_or(alpha[k], floor), floor);
+        sum += alpha[k];
+    }
+    if (!ffinite(sum) || sum <= 0.f) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) alpha[k] = (k == 0) ? 1.f : 0.f;
+        return;
+    }
+    const float inv = rcp(sum);
+#pragma unroll
+    for (int k = 0; k < M; ++k) alpha[k] *= inv;
+}
+
+// Volume fractions are stored as the reference's nmat-1 independent values,
+// except that for two materials the stored value is alpha_0 when
+// alpha_0 <= 1/2 and -alpha_1 otherwise: the smaller of alpha and 1-alpha
+// then keeps full FP32 relative precision (a cell that is 1-1e-9 air still
+// knows it holds 1e-9 helium).
+template<int M>
+__device__ __forceinline__ void decode_alpha(const float* a, float* alpha)
+{
+    if constexpr (M == 2) {
+        // the sign bit selects the branch: -0.0 encodes alpha_1 = 0
+        if (!__builtin_signbit(a[0])) {
+            alpha[0] = a[0];
+            alpha[1] = 1.f - a[0];
+        } else {
+            alpha[1] = -a[0];
+            alpha[0] = 1.f + a[0];
+        }
+    } else {
+        float sum = 0.f;
+#pragma unroll
+        for (int k = 0; k < M - 1; ++k) {
+            alpha[k] = a[k];
+            sum += alpha[k];
+        }
+        alpha[M - 1] = 1.f - sum;
+    }
+}
+
+// alpha: a sanitised full vector (non-negative, summing to one)
+template<int M>
+__device__ __forceinline__ void encode_alpha(const float* alpha, float* a)
+{
+    if constexpr (M == 2) {
+        a[0] = (alpha[0] <= 0.5f) ? alpha[0] : -alpha[1];
+    } else {
+#pragma unroll
+        for (int k = 0; k < M - 1; ++k) a[k] = alpha[k];
+    }
+}
+
+// Energy is carried shifted by the stiffening constant,
+//   E_hat = E - B(alpha),  B(alpha) = sum_k alpha_k gamma_k p_inf_k / (gamma_k - 1),
+// with alpha the decoded stored volume fractions, so that
+// rho e - B = sum_k alpha_k p/(gamma_k - 1) carries no cancellation between
+// terms of size gamma p_inf and the pressure (water: 7.8e8 J/m^3 vs 1e5 Pa).
+template<int M>
+__device__ __forceinline__ float bshift(const EosMixed& P, const float* from, const float* to)
+{
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < M; ++k) s += P.b[k] * (from[k] - to[k]);
+    return s;
+}
+
+template<int M>
+__device__ __forceinline__ float bdot(const EosMixed& P, const float* alpha)
+{
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < M; ++k) s += P.b[k] * alpha[k];
+    return s;
+}
+
+template<int M>
+__device__ __forceinline__ bool interface_cell(const float* alpha)
+{
+    bool itf = false;
+#pragma unroll
+    for (int k = 0; k < M; ++k) itf |= (alpha[k] > 1e-6f && alpha[k] < 1.f - 1e-6f);
+    return itf;
+}
+
+// p from the shifted internal energy density (rho e - B(alpha)) = sum_k alpha_k p/(gamma_k-1)
+template<int M>
+__device__ __forceinline__ float pressure_shifted(const EosMixed& P, float rho_e_hat, const float* alpha)
+{
+    float S = 0.f;
+#pragma unroll
+    for (int k = 0; k < M; ++k) S += fmaxs(alpha[k], 0.f) * P.inv_gm1[k];
+    return rho_e_hat * rcp(S);
+}
+
+template<int M>
+__device__ __forceinline__ float pressure_shifted_d(const EosMixed& P, double rho_e_hat, const float* alpha)
+{
+    double S = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) S += (double)fmaxs(alpha[k], 0.f) * P.inv_gm1_d[k];
+    return (float)(rho_e_hat / S);
+}
+
+// sum_k alpha_k max(p,1e-10)/(gamma_k-1): rho e - B at pressure p
+template<int M>
+__device__ __forceinline__ float energy_shifted(const EosMixed& P, float p, const float* alpha)
+{
+    const float pk = fmaxs(p, 1e-10f);
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < M; ++k) s += fmaxs(alpha[k], 0.f) * (pk * P.inv_gm1[k]);
+    return s;
+}
+
+// cons_to_prim (primitives.hpp:111-162) with the closed-form pressure
+template<int D, int M>
+__device__ __forceinline__ void cons_to_prim(const EosMixed& P, const St<D, M>& U, Pr<D, M>& R)
+{
+    float ad[M];
+    decode_alpha<M>(U.a, ad);
+#pragma unroll
+    for (int k = 0; k < M; ++k) R.alpha[k] = ad[k];
+    sanitise<M>(R.alpha, 1e-12f);
+    float rho = 0.f;
+#pragma unroll
+    for (int k = 0; k < M; ++k) rho += ffinite_or(U.m[k]);
+    rho = fmaxs(rho, 1e-12f);
+    const float inv_rho = rcp(fmaxs(rho, 1e-10f));
+    float v2 = 0.f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        R.vel[d] = ffinite_or(U.mom[d]) * inv_rho;
+        v2 += R.vel[d] * R.vel[d];
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) R.rho[k] = fmaxs(ffinite_or(U.m[k]) * rcp(fmaxs(R.alpha[k], 1e-10f)), 1e-12f);
+    const float kinetic = 0.5f * rho * v2;
+    const float shift = bshift<M>(P, ad, R.alpha);
+    float p;
+    if (interface_cell<M>(R.alpha)) {
+        double m2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) m2 += (double)ffinite_or(U.mom[d]) * (double)ffinite_or(U.mom[d]);
+        const double rho_e_hat = (double)ffinite_or(U.E, kinetic) + (double)shift - 0.5 * m2 / (double)rho;
+        p = pressure_shifted_d<M>(P, rho_e_hat, R.alpha);
+    } else {
+        p = pressure_shifted<M>(P, ffinite_or(U.E, kinetic) + shift - kinetic, R.alpha);
+    }
+    R.p = ffinite(p) ? fmaxs(p, 1e-10f) : 1e-10f;
+}
+
+// prim_to_cons (primitives.hpp:166-212)
+template<int D, int M>
+__device__ __forceinline__ void prim_to_cons(const EosMixed& P, const Pr<D, M>& Q, St<D, M>& U)
+{
+    float alpha[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) alpha[k] = Q.alpha[k];
+    sanitise<M>(alpha, 1e-12f);
+    float rho = 0.f;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        U.m[k] = alpha[k] * fmaxs(Q.rho[k], 1e-12f);
+        rho += U.m[k];
+    }
+    rho = fmaxs(rho, 1e-12f);
+    float v2 = 0.f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        U.mom[d] = rho * Q.vel[d];
+        v2 += Q.vel[d] * Q.vel[d];
+    }
+    encode_alpha<M>(alpha, U.a);
+    float ad[M];
+    decode_alpha<M>(U.a, ad);
+    U.E = energy_shifted<M>(P, fmaxs(Q.p, 1e-12f), alpha) + bshift<M>(P, alpha, ad) + 0.5f * rho * v2;
+}
+
+// repair_state (primitives.hpp:326-457) on a state whose partial masses are
+// of type MT (float for face/half states, double for stored cells)
+template<int D, int M, typename S>
+__device__ __forceinline__ void repair(const EosMixed& P, S& U)
+{
+    using MT = decltype(U.m[0] + 0.f);
+#pragma unroll
+    for (int k = 0; k < M; ++k) U.m[k] = ffinite(U.m[k]) ? (U.m[k] < MT(0) ? MT(0) : U.m[k]) : MT(0);
+#pragma unroll
+    for (int k = 0; k < M - 1; ++k) U.a[k] = ffinite_or(U.a[k]);
+#pragma unroll
+    for (int d = 0; d < D; ++d) U.mom[d] = ffinite_or(U.mom[d]);
+    U.E = ffinite_or(U.E);
+
+    MT rb = MT(0);
+#pragma unroll
+    for (int k = 0; k < M; ++k) rb += U.m[k];
+    const float rho_before = fmaxs((float)rb, 1e-12f);
+    const float inv_rb = rcp(fmaxs(rho_before, 1e-10f));
+    float vel_before[D];
+    float v2b = 0.f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        vel_before[d] = U.mom[d] * inv_rb;
+        v2b += vel_before[d] * vel_before[d];
+    }
+    float ad0[M], alpha[M], ad1[M];
+    decode_alpha<M>(U.a, ad0);
+#pragma unroll
+    for (int k = 0; k < M; ++k) alpha[k] = ad0[k];
+    sanitise<M>(alpha, 0.f);
+    encode_alpha<M>(alpha, U.a);
+    decode_alpha<M>(U.a, ad1);
+    // E_hat now refers to the re-encoded fractions
+    const float e_hat_before = U.E + bshift<M>(P, ad0, alpha);  // rho e - B(alpha) plus kinetic
+    U.E += bshift<M>(P, ad0, ad1);
+    bool removed = false;
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+        if (alpha[k] <= 1e-10f && U.m[k] > MT(1e-12)) {
+            U.m[k] = MT(0);
+            removed = true;
+        }
+    MT rt = MT(0);
+#pragma unroll
+    for (int k = 0; k < M; ++k) rt += U.m[k];
+    if (!(rt > MT(0))) {
+        int dom = 0;
+#pragma unroll
+        for (int k = 1; k < M; ++k)
+            if (alpha[k] > alpha[dom]) dom = k;
+#pragma unroll
+        for (int k = 0; k < M; ++k)
+            if (k == dom) U.m[k] = MT(1e-12);
+        rt = MT(1e-12);
+    }
+    const float rho = (float)rt;
+    const float fix = bshift<M>(P, alpha, ad1);
+    if (removed) {
+        const float p_before = fmaxs(pressure_shifted<M>(P, e_hat_before - 0.5f * rho_before * v2b, alpha), 1e-10f);
+#pragma unroll
+        for (int d = 0; d < D; ++d) U.mom[d] = rho * vel_before[d];
+        U.E = energy_shifted<M>(P, p_before, alpha) + fix + 0.5f * rho * v2b;
+    }
+    const float inv = rcp(fmaxs(rho, 1e-10f));
+    float v2 = 0.f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const float v = U.mom[d] * inv;
+        v2 += v * v;
+    }
+    float emin = 0.f;
+#pragma unroll
+    for (int k = 0; k < M; ++k) emin += alpha[k] * P.bmin[k];
+    U.E = fmaxs(U.E, 0.5f * rho * v2 + emin + fix);
+}
+
+// muscl.hpp:26-40
+template<int D, int M>
+__device__ __forceinline__ void enforce(Pr<D, M>& Q)
+{
+#pragma unroll
+    for (int k = 0; k < M; ++k) Q.rho[k] = fmaxs(Q.rho[k], 1e-12f);
+    sanitise<M>(Q.alpha, 0.f);
+    Q.p = fmaxs(Q.p, 1e-12f);
+}
+
+__device__ __forceinline__ float minmod(float a, float b)
+{
+    if (a * b <= 0.f) return 0.f;
+    return __builtin_fabsf(a) < __builtin_fabsf(b) ? a : b;
+}
+
+// muscl.hpp:43-112
+template<int D, int M>
+__device__ __forceinline__ void reconstruct(const Pr<D, M>& L, const Pr<D, M>& C, const Pr<D, M>& R, Pr<D, M>& lf,
+                                            Pr<D, M>& rf)
+{
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const float s = 0.5f * minmod(C.rho[k] - L.rho[k], R.rho[k] - C.rho[k]);
+        lf.rho[k] = C.rho[k] - s;
+        rf.rho[k] = C.rho[k] + s;
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const float s = 0.5f * minmod(C.alpha[k] - L.alpha[k], R.alpha[k] - C.alpha[k]);
+        lf.alpha[k] = C.alpha[k] - s;
+        rf.alpha[k] = C.alpha[k] + s;
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const float s = 0.5f * minmod(C.vel[d] - L.vel[d], R.vel[d] - C.vel[d]);
+        lf.vel[d] = C.vel[d] - s;
+        rf.vel[d] = C.vel[d] + s;
+    }
+    const float s = 0.5f * minmod(C.p - L.p, R.p - C.p);
+    lf.p = C.p - s;
+    rf.p = C.p + s;
+    enforce<D, M>(lf);
+    enforce<D, M>(rf);
+}
+
+// sum_k alpha_k / (rho_k c_k^2) with rho_k c_k^2 = gamma_k (p + p_inf_k) for
+// ideal and stiffened gases (the Wood / Kapila compressibility)
+template<int M>
+__device__ __forceinline__ float compressibility(const EosMixed& P, float p, const float* alpha, float* imp,
+                                                 bool* active, float floor_alpha)
+{
+    float beta = 0.f;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const float a = fmaxs(ffinite_or(alpha[k]), 0.f);
+        imp[k] = fmaxs(P.gamma[k] * (p + P.pinf[k]), 1e-12f);
+        active[k] = !(a <= floor_alpha);
+        if (active[k]) beta += a * rcp(imp[k]);
+    }
+    return beta;
+}
+
+// Wood sound speed (eos.hpp:195-221) and the allaire / equal_velocity
+// variants (eos.hpp:223-275), closed form for ideal and stiffened gases
+template<int M>
+__device__ __forceinline__ float sound_speed(const EosMixed& P, float rho, float p, const float* alpha, const float* rho_k)
+{
+    rho = fmaxs(rho, 1e-10f);
+    p = fmaxs(p, 1e-10f);
+    if (P.lambda_model == LAMBDA_KAPILA) {
+        float imp[M];
+        bool act[M];
+        const float beta = compressibility<M>(P, p, alpha, imp, act, -1.f);
+        return frsq(rho * fmaxs(beta, 1e-14f));
+    }
+    float cs[M], xi_mix = 0.f, inv_sum = 0.f;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const float a = fmaxs(ffinite_or(alpha[k]), 0.f);
+        cs[k] = fsqrt(P.gamma[k] * (p + P.pinf[k]) * rcp(fmaxs(rho_k[k], 1e-10f)));
+        xi_mix += a * P.inv_gm1[k];
+        inv_sum += a * rcp(fmaxs(cs[k], 1e-14f));
+    }
+    float rc2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const float a = fmaxs(ffinite_or(alpha[k]), 0.f);
+        const float lam = (P.lambda_model == LAMBDA_EQUAL_VELOCITY) ? rcp(fmaxs(cs[k], 1e-14f) * inv_sum) : 1.f;
+        rc2 += lam * a * P.inv_gm1[k] * cs[k] * cs[k];
+    }
+    return fsqrt(fmaxs(rc2 * rcp(xi_mix) * rcp(rho), 1e-14f));
+}
+
+// alpha_rhs_coefficients (primitives.hpp:216-322)
+template<int D, int M>
+__device__ __forceinline__ void alpha_rhs(const EosMixed& P, const Pr<D, M>& Q, float* rhs)
+{
+#pragma unroll
+    for (int k = 0; k < M; ++k) rhs[k] = Q.alpha[k];
+    sanitise<M>(rhs, 0.f);
+    if (P.lambda_model == LAMBDA_ALLAIRE) return;
+    const float p = fmaxs(ffinite_or(Q.p), 1e-12f);
+    const float srcf = fmaxs(P.alpha_source_floor, 0.f);
+    if (P.lambda_model == LAMBDA_EQUAL_VELOCITY) {
+        float cs[M], inv_sum = 0.f;
+        bool act[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const float a = fmaxs(ffinite_or(rhs[k]), 0.f);
+            act[k] = !(a <= 1e-12f || a < srcf);
+            cs[k] = act[k] ? fsqrt(P.gamma[k] * (p + P.pinf[k]) * rcp(fmaxs(Q.rho[k], 1e-12f))) : 1e-12f;
+            if (act[k]) inv_sum += a * rcp(cs[k]);
+        }
+        if (inv_sum <= 1e-12f || !ffinite(inv_sum)) return;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const float a = fmaxs(ffinite_or(rhs[k]), 0.f);
+            rhs[k] = (a <= 1e-12f) ? 0.f : a * rcp(cs[k] * inv_sum);
+        }
+        return;
+    }
+    float imp[M];
+    bool act[M];
+    float beta = 0.f;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const float a = fmaxs(ffinite_or(rhs[k]), 0.f);
+        act[k] = !(a <= 1e-12f || a < srcf);
+        imp[k] = fmaxs(P.gamma[k] * (p + P.pinf[k]), 1e-12f);
+        if (act[k]) beta += a * rcp(imp[k]);
+    }
+    if (beta <= 1e-12f || !ffinite(beta)) return;
+    const float q = rcp(beta);
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const float a = fmaxs(ffinite_or(rhs[k]), 0.f);
+        rhs[k] = act[k] ? a + a * (q * rcp(imp[k]) - 1.f) : 0.f;
+    }
+}
+
+// physical flux (flux.hpp:11-43) with the true energy E = E_hat + B(alpha)
+template<int D, int M, int DIR>
+__device__ __forceinline__ void flux_normal(const EosMixed& P, const St<D, M>& U, const Pr<D, M>& Q, Fx<D, M>& F)
+{
+    float ad[M];
+    decode_alpha<M>(U.a, ad);
+    float rho = 0.f;
+#pragma unroll
+    for (int k = 0; k < M; ++k) rho += U.m[k];
+    const float un = Q.vel[DIR];
+#pragma unroll
+    for (int k = 0; k < M; ++k) F.m[k] = U.m[k] * un;
+#pragma unroll
+    for (int d = 0; d < D; ++d) F.mom[d] = rho * Q.vel[d] * un + (d == DIR ? Q.p : 0.f);
+    F.E = (U.E + bdot<M>(P, ad) + Q.p) * un;
+}
+
+// muscl.hpp:143-191
+template<int D, int M, int DIR>
+__device__ __forceinline__ void hancock(const EosMixed& P, const St<D, M>& Uc, const Pr<D, M>& Pc, const St<D, M>& ul,
+                                        const St<D, M>& ur, const Pr<D, M>& lf, const Pr<D, M>& rf, float dt, float dx,
+                                        St<D, M>& out)
+{
+    Fx<D, M> FL, FR;
+    flux_normal<D, M, DIR>(P, ul, lf, FL);
+    flux_normal<D, M, DIR>(P, ur, rf, FR);
+    const float inv_dx = rcp(dx);
+    const float factor = 0.5f * dt * inv_dx;
+    out = Uc;
+#pragma unroll
+    for (int k = 0; k < M; ++k) out.m[k] -= factor * (FR.m[k] - FL.m[k]);
+#pragma unroll
+    for (int d = 0; d < D; ++d) out.mom[d] -= factor * (FR.mom[d] - FL.mom[d]);
+    out.E -= factor * (FR.E - FL.E);
+    const float ul_ = lf.vel[DIR], ur_ = rf.vel[DIR];
+    const float div_u = (ur_ - ul_) * inv_dx;
+    float rhs[M], an[M];
+    alpha_rhs<D, M>(P, Pc, rhs);
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+        an[k] = Pc.alpha[k] - factor * (rf.alpha[k] * ur_ - lf.alpha[k] * ul_) + 0.5f * dt * rhs[k] * div_u;
+    sanitise<M>(an, 0.f);
+    float ad_old[M], ad_new[M];
+    decode_alpha<M>(out.a, ad_old);
+    encode_alpha<M>(an, out.a);
+    decode_alpha<M>(out.a, ad_new);
+    out.E += bshift<M>(P, ad_old, ad_new);
+    repair<D, M>(P, out);
+}
+
+// hllc.hpp:62-195
+template<int D, int M, int DIR>
+__device__ __forceinline__ bool hllc(const EosMixed& P, const St<D, M>& UL, const St<D, M>& UR, const Pr<D, M>& PL,
+                                     const Pr<D, M>& PR, Fx<D, M>& F, float& fv)
+{
+    float rhoL = 0.f, rhoR = 0.f;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        rhoL += UL.m[k];
+        rhoR += UR.m[k];
+    }
+    if (!ffinite(rhoL) || !ffinite(rhoR) || !ffinite(PL.p) || !ffinite(PR.p)) return false;
+    rhoL = rhoL <= 0.f ? 1e-10f : rhoL;
+    rhoR = rhoR <= 0.f ? 1e-10f : rhoR;
+    const float pL = PL.p <= 0.f ? 1e-10f : PL.p;
+    const float pR = PR.p <= 0.f ? 1e-10f : PR.p;
+    const float unL = PL.vel[DIR], unR = PR.vel[DIR];
+    const float cL = sound_speed<M>(P, rhoL, pL, PL.alpha, PL.rho);
+    const float cR = sound_speed<M>(P, rhoR, pR, PR.alpha, PR.rho);
+    const float sL = fmins(unL - cL, unR - cR);
+    const float sR = fmaxs(unL + cL, unR + cR);
+    const float num = (pR - pL) + rhoL * unL * (sL - unL) - rhoR * unR * (sR - unR);
+    const float den = rhoL * (sL - unL) - rhoR * (sR - unR);
+    float s_star = safe_div_f(num, den);
+    if (__builtin_fabsf(pL - pR) < 1e-10f && __builtin_fabsf(unL - unR) < 1e-10f) s_star = unL;
+    Fx<D, M> FL, FR;
+    flux_normal<D, M, DIR>(P, UL, PL, FL);
+    flux_normal<D, M, DIR>(P, UR, PR, FR);
+    const bool average = !ffinite(sL) || !ffinite(sR) || !ffinite(s_star) ||
+                         (!(0.f <= sL) && !(sR <= 0.f) && !(sL < sR));
+    if (average) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) F.m[k] = 0.5f * (FL.m[k] + FR.m[k]);
+#pragma unroll
+        for (int d = 0; d < D; ++d) F.mom[d] = 0.5f * (FL.mom[d] + FR.mom[d]);
+        F.E = 0.5f * (FL.E + FR.E);
+        fv = 0.5f * (unL + unR);
+        return true;
+    }
+    if (0.f <= sL) {
+        F = FL;
+        fv = unL;
+        return true;
+    }
+    if (sR <= 0.f) {
+        F = FR;
+        fv = unR;
+        return true;
+    }
+    const bool left = (0.f <= s_star);
+    const St<D, M>& UK = left ? UL : UR;
+    const Pr<D, M>& PK = left ? PL : PR;
+    const Fx<D, M>& FK = left ? FL : FR;
+    const float rhoK = left ? rhoL : rhoR;
+    const float pK = left ? pL : pR;
+    const float sK = left ? sL : sR;
+    const float unK = left ? unL : unR;
+    // star state (hllc.hpp:15-41, 144-194) written through
+    // delta = rho*/rho_K - 1 = (s* - u_K)/(s_K - s*) so that E* - E_K carries
+    // no cancellation against the stiffened-gas energy
+    float adK[M];
+    decode_alpha<M>(UK.a, adK);
+    const float EK = UK.E + bdot<M>(P, adK);
+    const float ratio = safe_div_f(sK - unK, sK - s_star);
+    const float rho_star = rhoK * ratio;
+    const float delta = safe_div_f(s_star - unK, sK - s_star);
+    const float e_star_term = rho_star * (s_star - unK) * (s_star + safe_div_f(pK, rhoK * (sK - unK)));
+    const float E_star = rho_star * safe_div_f(EK, rhoK) + e_star_term;
+    fv = s_star;
+    if (rho_star <= 0.f || !ffinite(rho_star) || !ffinite(E_star)) {
+        float adL[M], adR[M];
+        decode_alpha<M>(UL.a, adL);
+        decode_alpha<M>(UR.a, adR);
+        const float inv = rcp(sR - sL);
+        const float sLsR = sL * sR;
+#pragma unroll
+        for (int k = 0; k < M; ++k) F.m[k] = inv * ((sR * FL.m[k] - sL * FR.m[k]) + sLsR * (UR.m[k] - UL.m[k]));
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            F.mom[d] = inv * ((sR * FL.mom[d] - sL * FR.mom[d]) + sLsR * (UR.mom[d] - UL.mom[d]));
+        F.E = inv * ((sR * FL.E - sL * FR.E) + sLsR * ((UR.E - UL.E) + bshift<M>(P, adR, adL)));
+        return true;
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) F.m[k] = FK.m[k] + sK * (UK.m[k] * delta);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const float u_star = (d == DIR) ? s_star : PK.vel[d];
+        F.mom[d] = FK.mom[d] + sK * (rho_star * u_star - UK.mom[d]);
+    }
+    F.E = FK.E + sK * (EK * delta + e_star_term);
+    return true;
+}
+
+}  // namespace f32
+
+struct Fp32Mixed {
+    using R = float;
+    using Eos = EosMixed;
+    template<int D, int M> using St = f32::St<D, M>;
+    template<int D, int M> using Pr = f32::Pr<D, M>;
+    template<int D, int M> using Fx = f32::Fx<D, M>;
+    template<int D, int M> using Cell = f32::Cell<D, M>;
+
+    // storage: M double planes (partial masses) then D + M float planes
+    // (momentum, energy, independent volume fractions)
+    template<int D, int M>
+    static __device__ __forceinline__ void load(const void* base, long long pitch, long long c, Cell<D, M>& U)
+    {
+        const double* qm = (const double*)base;
+        const float* qf = (const float*)(qm + M * pitch);
+#pragma unroll
+        for (int k = 0; k < M; ++k) U.m[k] = qm[k * pitch + c];
+        int f = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) U.mom[d] = qf[(f++) * pitch + c];
+        U.E = qf[(f++) * pitch + c];
+#pragma unroll
+        for (int k = 0; k < M - 1; ++k) U.a[k] = qf[(f++) * pitch + c];
+    }
+
+    template<int D, int M>
+    static __device__ __forceinline__ void store(void* base, long long pitch, long long c, const Cell<D, M>& U)
+    {
+        double* qm = (double*)base;
+        float* qf = (float*)(qm + M * pitch);
+#pragma unroll
+        for (int k = 0; k < M; ++k) qm[k * pitch + c] = U.m[k];
+        int f = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) qf[(f++) * pitch + c] = U.mom[d];
+        qf[(f++) * pitch + c] = U.E;
+#pragma unroll
+        for (int k = 0; k < M - 1; ++k) qf[(f++) * pitch + c] = U.a[k];
+    }
+
+    // packed FP64 cell -> storage: volume fractions clamped to [0,1] and
+    // encoded, energy shifted by B(alpha) in FP64
+    template<int D, int M>
+    static __device__ __forceinline__ void from_packed(const Eos& P, const double* p, Cell<D, M>& U)
+    {
+        int o = 0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) U.m[k] = p[o++];
+#pragma unroll
+        for (int d = 0; d < D; ++d) U.mom[d] = (float)p[o++];
+        const double E = p[o++];
+        double al[M];
+        double sum = 0.0;
+#pragma unroll
+        for (int k = 0; k < M - 1; ++k) {
+            al[k] = p[o++];
+            sum += al[k];
+        }
+        al[M - 1] = 1.0 - sum;
+        if constexpr (M == 2) {
+            const double a0 = al[0] < 0.0 ? 0.0 : (al[0] > 1.0 ? 1.0 : al[0]);
+            U.a[0] = (a0 <= 0.5) ? (float)a0 : -(float)(1.0 - a0);
+        } else {
+#pragma unroll
+            for (int k = 0; k < M - 1; ++k) U.a[k] = (float)al[k];
+        }
+        U.E = (float)(E - Bd<M>(P, U.a));
+    }
+
+    // B(decoded alpha) in FP64
+    template<int M>
+    static __device__ __forceinline__ double Bd(const Eos& P, const float* a)
+    {
+        double al[M];
+        if constexpr (M == 2) {
+            const bool pos = !__builtin_signbit(a[0]);
+            al[0] = pos ? (double)a[0] : 1.0 + (double)a[0];
+            al[1] = pos ? 1.0 - (double)a[0] : -(double)a[0];
+        } else {
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < M - 1; ++k) {
+                al[k] = a[k];
+                sum += al[k];
+            }
+            al[M - 1] = 1.0 - sum;
+        }
+        double b = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) b += P.b_d[k] * al[k];
+        return b;
+    }
+
+    template<int D, int M>
+    static __device__ __forceinline__ void to_packed(const Eos& P, const Cell<D, M>& U, double* p)
+    {
+        int o = 0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) p[o++] = U.m[k];
+#pragma unroll
+        for (int d = 0; d < D; ++d) p[o++] = U.mom[d];
+        p[o++] = (double)U.E + Bd<M>(P, U.a);
+        if constexpr (M == 2) {
+            p[o++] = !__builtin_signbit(U.a[0]) ? (double)U.a[0] : 1.0 + (double)U.a[0];
+        } else {
+#pragma unroll
+            for (int k = 0; k < M - 1; ++k) p[o++] = U.a[k];
+        }
+    }
+
+    template<int D, int M>
+    static __device__ __forceinline__ void to_state(const Cell<D, M>& c, St<D, M>& U)
+    {
+#pragma unroll
+        for (int k = 0; k < M; ++k) U.m[k] = (float)c.m[k];
+#pragma unroll
+        for (int d = 0; d < D; ++d) U.mom[d] = c.mom[d];
+        U.E = c.E;
+#pragma unroll
+        for (int k = 0; k < M - 1; ++k) U.a[k] = c.a[k];
+    }
+
+    template<int D, int M>
+    static __device__ __forceinline__ void cons_to_prim(const Eos& P, const St<D, M>& U, Pr<D, M>& Q)
+    {
+        f32::cons_to_prim<D, M>(P, U, Q);
+    }
+    template<int D, int M>
+    static __device__ __forceinline__ void enforce(Pr<D, M>& Q) { f32::enforce<D, M>(Q); }
+    template<int D, int M>
+    static __device__ __forceinline__ void reconstruct(const Pr<D, M>& L, const Pr<D, M>& C, const Pr<D, M>& Rt,
+                                                       Pr<D, M>& lf, Pr<D, M>& rf)
+    {
+        f32::reconstruct<D, M>(L, C, Rt, lf, rf);
+    }
+    template<int D, int M>
+    static __device__ __forceinline__ void prim_to_cons(const Eos& P, const Pr<D, M>& Q, St<D, M>& U)
+    {
+        f32::prim_to_cons<D, M>(P, Q, U);
+    }
+    template<int D, int M>
+    static __device__ __forceinline__ void repair(const Eos& P, St<D, M>& U) { f32::repair<D, M>(P, U); }
+    template<int D, int M, int DIR>
+    static __device__ __forceinline__ void hancock(const Eos& P, const St<D, M>& Uc, const Pr<D, M>& Pc,
+                                                   const St<D, M>& ul, const St<D, M>& ur, const Pr<D, M>& lf,
+                                                   const Pr<D, M>& rf, double dt, double dx, St<D, M>& out)
+    {
+        f32::hancock<D, M, DIR>(P, Uc, Pc, ul, ur, lf, rf, (float)dt, (float)dx, out);
+    }
+    template<int D, int M, int DIR>
+    static __device__ __forceinline__ bool hllc(const Eos& P, const St<D, M>& UL, const St<D, M>& UR,
+                                                const Pr<D, M>& PL, const Pr<D, M>& PR, Fx<D, M>& F, float& fv)
+    {
+        return f32::hllc<D, M, DIR>(P, UL, UR, PL, PR, F, fv);
+    }
+
+    template<int D, int M>
+    static __device__ __forceinline__ void update(const Eos& P, const Cell<D, M>& Uin, const Pr<D, M>& Pn,
+                                                  const Fx<D, M>& Fl, const Fx<D, M>& Fr, float fvl, float fvr,
+                                                  const float* afl, const float* afr, double dt, double dx,
+                                                  Cell<D, M>& r)
+    {
+        const double lambda = dt / dx;
+        const float lf = (float)lambda;
+        r = Uin;
+#pragma unroll
+        for (int k = 0; k < M; ++k) r.m[k] -= lambda * ((double)Fr.m[k] - (double)Fl.m[k]);
+#pragma unroll
+        for (int d = 0; d < D; ++d) r.mom[d] -= lf * (Fr.mom[d] - Fl.mom[d]);
+        r.E -= lf * (Fr.E - Fl.E);
+        f32::repair<D, M>(P, r);
+        float rhs[M], an[M];
+        f32::alpha_rhs<D, M>(P, Pn, rhs);
+        const float div_u = (fvr - fvl) * (float)(1.0 / dx);
+        const float dtf = (float)dt;
+#pragma unroll
+        for (int k = 0; k < M; ++k) an[k] = Pn.alpha[k] - lf * (afr[k] - afl[k]) + dtf * rhs[k] * div_u;
+        f32::sanitise<M>(an, 0.f);
+        float ad_old[M], ad_new[M];
+        f32::decode_alpha<M>(r.a, ad_old);
+        f32::encode_alpha<M>(an, r.a);
+        f32::decode_alpha<M>(r.a, ad_new);
+        r.E += f32::bshift<M>(P, ad_old, ad_new);
+        f32::repair<D, M>(P, r);
+    }
+
+    template<int D, int M>
+    static __device__ __forceinline__ void accumulate(const Eos& P, Cell<D, M>& acc, const Cell<D, M>& r,
+                                                      const Cell<D, M>& U)
+    {
+#pragma unroll
+        for (int k = 0; k < M; ++k) acc.m[k] += r.m[k] - U.m[k];
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc.mom[d] += r.mom[d] - U.mom[d];
+        float aa[M], ar[M], au[M], sum[M], dn[M];
+        f32::decode_alpha<M>(acc.a, aa);
+        f32::decode_alpha<M>(r.a, ar);
+        f32::decode_alpha<M>(U.a, au);
+#pragma unroll
+        for (int k = 0; k < M; ++k) sum[k] = aa[k] + (ar[k] - au[k]);
+        if constexpr (M == 2) {
+            float cl[2] = {sum[0], sum[1]};
+            if (!(cl[0] >= 0.f)) {
+                cl[0] = 0.f;
+                cl[1] = 1.f;
+            } else if (!(cl[1] >= 0.f)) {
+                cl[0] = 1.f;
+                cl[1] = 0.f;
+            }
+            f32::encode_alpha<M>(cl, acc.a);
+        } else {
+#pragma unroll
+            for (int k = 0; k < M - 1; ++k) acc.a[k] = sum[k];
+        }
+        f32::decode_alpha<M>(acc.a, dn);
+        acc.E += (r.E - U.E) + f32::bshift<M>(P, sum, dn);
+        f32::repair<D, M>(P, acc);
+    }
+
+    template<int D, int M>
+    static __device__ __forceinline__ void repair_cell(const Eos& P, Cell<D, M>& U) { f32::repair<D, M>(P, U); }
+
+    // FP64 candidate cfl*dx/(|u|+c) from the FP32 wave speed
+    template<int D, int M>
+    static __device__ __forceinline__ double cfl_candidate(const Eos& P, const Cell<D, M>& C, const CflArgs& a)
+    {
+        St<D, M> U;
+        to_state<D, M>(C, U);
+        Pr<D, M> Q;
+        f32::cons_to_prim<D, M>(P, U, Q);
+        float rho = 0.f;
+#pragma unroll
+        for (int k = 0; k < M; ++k) rho += U.m[k];
+        const double c = (double)f32::sound_speed<M>(P, rho, Q.p, Q.alpha, Q.rho);
+        double best = bits_to_double(kInfBits);
+        if (!a.unsplit) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const double ws = (double)__builtin_fabsf(Q.vel[d]) + c;
+                if (ws > 1e-14) best = smin(best, a.cfl_dx[d] / ws);
+            }
+        } else {
+            double spectral = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) spectral += ((double)__builtin_fabsf(Q.vel[d]) + c) / a.dx[d];
+            if (spectral > 1e-14) best = smin(best, a.cfl / spectral);
+        }
+        return best;
+    }
+
+    template<int D, int M>
+    static __device__ __forceinline__ double mass(const Cell<D, M>& U, int k) { return U.m[k]; }
+    template<int D, int M>
+    static __device__ __forceinline__ double mom(const Cell<D, M>& U, int d) { return (double)U.mom[d]; }
+    template<int D, int M>
+    static __device__ __forceinline__ double energy(const Eos& P, const Cell<D, M>& U) { return (double)U.E + Bd<M>(P, U.a); }
+};
+
+}  // namespace dimhip

@@ -1,0 +1,201 @@
+// policy_fp64.hpp — the FP64 arithmetic policy of the generic DIM kernels:
+// every operation is the reference's (dim_fp64.hpp), state stored as FP64
+// SoA planes in the reference's field order.
+#pragma once
+
+#include "dim_fp64.hpp"
+
+namespace dimhip {
+
+struct CflArgs {
+    double cfl_dx[3];
+    double dx[3];
+    double cfl;
+    int unsplit;
+};
+
+constexpr unsigned long long kInfBits = 0x7FF0000000000000ull;
+__device__ __forceinline__ double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ unsigned long long double_to_bits(double x) { return (unsigned long long)__double_as_longlong(x); }
+
+struct Fp64 {
+    using R = double;
+    using Eos = EosParams;
+    template<int D, int M> using St = State<D, M>;
+    template<int D, int M> using Pr = Prim<D, M>;
+    template<int D, int M> using Fx = Flux<D, M>;
+    template<int D, int M> using Cell = State<D, M>;
+
+    // storage: nv double planes, field f of cell c at q[f * pitch + c]
+    template<int D, int M>
+    static __device__ __forceinline__ void load(const void* base, long long pitch, long long c, Cell<D, M>& U)
+    {
+        const double* q = (const double*)base;
+        int f = 0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) U.m[k] = q[(f++) * pitch + c];
+#pragma unroll
+        for (int d = 0; d < D; ++d) U.mom[d] = q[(f++) * pitch + c];
+        U.E = q[(f++) * pitch + c];
+#pragma unroll
+        for (int k = 0; k < M - 1; ++k) U.a[k] = q[(f++) * pitch + c];
+    }
+
+    template<int D, int M>
+    static __device__ __forceinline__ void store(void* base, long long pitch, long long c, const Cell<D, M>& U)
+    {
+        double* q = (double*)base;
+        int f = 0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) q[(f++) * pitch + c] = U.m[k];
+#pragma unroll
+        for (int d = 0; d < D; ++d) q[(f++) * pitch + c] = U.mom[d];
+        q[(f++) * pitch + c] = U.E;
+#pragma unroll
+        for (int k = 0; k < M - 1; ++k) q[(f++) * pitch + c] = U.a[k];
+    }
+
+    // packed per-cell doubles <-> Cell
+    template<int D, int M>
+    static __device__ __forceinline__ void from_packed(const Eos&, const double* p, Cell<D, M>& U)
+    {
+        int o = 0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) U.m[k] = p[o++];
+#pragma unroll
+        for (int d = 0; d < D; ++d) U.mom[d] = p[o++];
+        U.E = p[o++];
+#pragma unroll
+        for (int k = 0; k < M - 1; ++k) U.a[k] = p[o++];
+    }
+
+    template<int D, int M>
+    static __device__ __forceinline__ void to_packed(const Eos&, const Cell<D, M>& U, double* p)
+    {
+        int o = 0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) p[o++] = U.m[k];
+#pragma unroll
+        for (int d = 0; d < D; ++d) p[o++] = U.mom[d];
+        p[o++] = U.E;
+#pragma unroll
+        for (int k = 0; k < M - 1; ++k) p[o++] = U.a[k];
+    }
+
+    template<int D, int M>
+    static __device__ __forceinline__ void to_state(const Cell<D, M>& c, St<D, M>& U) { U = c; }
+
+    template<int D, int M>
+    static __device__ __forceinline__ void cons_to_prim(const Eos& P, const St<D, M>& U, Pr<D, M>& Q)
+    {
+        dimhip::cons_to_prim<D, M>(P, U, Q);
+    }
+    template<int D, int M>
+    static __device__ __forceinline__ void enforce(Pr<D, M>& Q) { enforce_positive_primitive<D, M>(Q); }
+    template<int D, int M>
+    static __device__ __forceinline__ void reconstruct(const Pr<D, M>& L, const Pr<D, M>& C, const Pr<D, M>& Rt,
+                                                       Pr<D, M>& lf, Pr<D, M>& rf)
+    {
+        reconstruct_cell_faces<D, M>(L, C, Rt, lf, rf);
+    }
+    template<int D, int M>
+    static __device__ __forceinline__ void prim_to_cons(const Eos& P, const Pr<D, M>& Q, St<D, M>& U)
+    {
+        dimhip::prim_to_cons<D, M>(P, Q, U);
+    }
+    template<int D, int M>
+    static __device__ __forceinline__ void repair(const Eos& P, St<D, M>& U) { repair_state<D, M>(P, U); }
+    template<int D, int M, int DIR>
+    static __device__ __forceinline__ void hancock(const Eos& P, const St<D, M>& Uc, const Pr<D, M>& Pc,
+                                                   const St<D, M>& ul, const St<D, M>& ur, const Pr<D, M>& lf,
+                                                   const Pr<D, M>& rf, double dt, double dx, St<D, M>& out)
+    {
+        hancock_predict<D, M, DIR>(P, Uc, Pc, ul, ur, lf, rf, dt, dx, out);
+    }
+    template<int D, int M, int DIR>
+    static __device__ __forceinline__ bool hllc(const Eos& P, const St<D, M>& UL, const St<D, M>& UR,
+                                                const Pr<D, M>& PL, const Pr<D, M>& PR, Fx<D, M>& F, double& fv)
+    {
+        return hllc_flux<D, M, DIR>(P, UL, UR, PL, PR, F, fv);
+    }
+
+    // solve_line's update of one cell (sweep_core.hpp:101-148)
+    template<int D, int M>
+    static __device__ __forceinline__ void update(const Eos& P, const Cell<D, M>& Uin, const Pr<D, M>& Pn,
+                                                  const Fx<D, M>& Fl, const Fx<D, M>& Fr, double fvl, double fvr,
+                                                  const double* afl, const double* afr, double dt, double dx,
+                                                  Cell<D, M>& r)
+    {
+        const double lambda = dt / dx;
+        r = Uin;
+#pragma unroll
+        for (int k = 0; k < M; ++k) r.m[k] -= lambda * (Fr.m[k] - Fl.m[k]);
+#pragma unroll
+        for (int d = 0; d < D; ++d) r.mom[d] -= lambda * (Fr.mom[d] - Fl.mom[d]);
+        r.E -= lambda * (Fr.E - Fl.E);
+        repair_state<D, M>(P, r);
+        double rhs[M], an[M];
+        alpha_rhs_coefficients<D, M>(P, Pn, rhs);
+        const double div_u = (fvr - fvl) / dx;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            an[k] = Pn.alpha[k];
+            an[k] -= lambda * (afr[k] - afl[k]);
+            an[k] += dt * rhs[k] * div_u;
+        }
+        sanitise_alpha<M>(an, 0.0);
+#pragma unroll
+        for (int k = 0; k < M - 1; ++k) r.a[k] = an[k];
+        repair_state<D, M>(P, r);
+    }
+
+    // accumulate_delta (advance_step.hpp:198-222): acc += U_dir - U_base; repair
+    template<int D, int M>
+    static __device__ __forceinline__ void accumulate(const Eos& P, Cell<D, M>& acc, const Cell<D, M>& r,
+                                                      const Cell<D, M>& U)
+    {
+#pragma unroll
+        for (int k = 0; k < M; ++k) acc.m[k] += r.m[k] - U.m[k];
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc.mom[d] += r.mom[d] - U.mom[d];
+        acc.E += r.E - U.E;
+#pragma unroll
+        for (int k = 0; k < M - 1; ++k) acc.a[k] += r.a[k] - U.a[k];
+        repair_state<D, M>(P, acc);
+    }
+
+    template<int D, int M>
+    static __device__ __forceinline__ void repair_cell(const Eos& P, Cell<D, M>& U) { repair_state<D, M>(P, U); }
+
+    // CFL candidate (cfl.hpp:46-63 / 94-116)
+    template<int D, int M>
+    static __device__ __forceinline__ double cfl_candidate(const Eos& P, const Cell<D, M>& U, const CflArgs& a)
+    {
+        Prim<D, M> Q;
+        dimhip::cons_to_prim<D, M>(P, U, Q);
+        const double c = mixture_sound_speed<M>(P, total_density<D, M>(U), Q.p, Q.alpha, Q.rho);
+        double best = bits_to_double(kInfBits);
+        if (!a.unsplit) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const double ws = __builtin_fabs(Q.vel[d]) + c;
+                if (ws > 1e-14) best = smin(best, a.cfl_dx[d] / ws);
+            }
+        } else {
+            double spectral = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) spectral += (__builtin_fabs(Q.vel[d]) + c) / a.dx[d];
+            if (spectral > 1e-14) best = smin(best, a.cfl / spectral);
+        }
+        return best;
+    }
+
+    template<int D, int M>
+    static __device__ __forceinline__ double mass(const Cell<D, M>& U, int k) { return U.m[k]; }
+    template<int D, int M>
+    static __device__ __forceinline__ double mom(const Cell<D, M>& U, int d) { return U.mom[d]; }
+    template<int D, int M>
+    static __device__ __forceinline__ double energy(const Eos&, const Cell<D, M>& U) { return U.E; }
+};
+
+}  // namespace dimhip

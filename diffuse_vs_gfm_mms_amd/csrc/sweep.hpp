@@ -1,0 +1,578 @@
+// sweep.hpp — the DIM kernels, generic over an arithmetic policy
+// (Fp64: the reference's arithmetic bit for bit; Fp32Mixed: FP32 bulk with
+// FP64 masses, reductions and interface-cell pressure recovery).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "policy_fp64.hpp"
+
+namespace dimhip {
+
+// LDS record helpers: a per-thread record stored as SoA planes sh[f * S + slot]
+template<typename R, int D, int M, template<int, int> class PrT>
+__device__ __forceinline__ void put_prim(R* sh, int S, int slot, int off, const PrT<D, M>& P)
+{
+    int f = off;
+#pragma unroll
+    for (int k = 0; k < M; ++k) sh[(f++) * S + slot] = P.rho[k];
+#pragma unroll
+    for (int k = 0; k < M; ++k) sh[(f++) * S + slot] = P.alpha[k];
+#pragma unroll
+    for (int d = 0; d < D; ++d) sh[(f++) * S + slot] = P.vel[d];
+    sh[(f++) * S + slot] = P.p;
+}
+
+template<typename R, int D, int M, template<int, int> class PrT>
+__device__ __forceinline__ void get_prim(const R* sh, int S, int slot, int off, PrT<D, M>& P)
+{
+    int f = off;
+#pragma unroll
+    for (int k = 0; k < M; ++k) P.rho[k] = sh[(f++) * S + slot];
+#pragma unroll
+    for (int k = 0; k < M; ++k) P.alpha[k] = sh[(f++) * S + slot];
+#pragma unroll
+    for (int d = 0; d < D; ++d) P.vel[d] = sh[(f++) * S + slot];
+    P.p = sh[(f++) * S + slot];
+}
+
+template<typename R, int D, int M, template<int, int> class StT>
+__device__ __forceinline__ void put_state(R* sh, int S, int slot, int off, const StT<D, M>& U)
+{
+    int f = off;
+#pragma unroll
+    for (int k = 0; k < M; ++k) sh[(f++) * S + slot] = U.m[k];
+#pragma unroll
+    for (int d = 0; d < D; ++d) sh[(f++) * S + slot] = U.mom[d];
+    sh[(f++) * S + slot] = U.E;
+#pragma unroll
+    for (int k = 0; k < M - 1; ++k) sh[(f++) * S + slot] = U.a[k];
+}
+
+template<typename R, int D, int M, template<int, int> class StT>
+__device__ __forceinline__ void get_state(const R* sh, int S, int slot, int off, StT<D, M>& U)
+{
+    int f = off;
+#pragma unroll
+    for (int k = 0; k < M; ++k) U.m[k] = sh[(f++) * S + slot];
+#pragma unroll
+    for (int d = 0; d < D; ++d) U.mom[d] = sh[(f++) * S + slot];
+    U.E = sh[(f++) * S + slot];
+#pragma unroll
+    for (int k = 0; k < M - 1; ++k) U.a[k] = sh[(f++) * S + slot];
+}
+
+// Block-wide min of positive doubles (or +inf) and one atomic per block on
+// the bit pattern (positive doubles order like unsigned integers).
+template<int NT>
+__device__ __forceinline__ void block_min_atomic(double v, unsigned long long* target)
+{
+    __shared__ double red[NT / 64 > 0 ? NT / 64 : 1];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = smin(v, o);
+    }
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x + threadIdx.z * blockDim.x * blockDim.y;
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+        double m = red[0];
+#pragma unroll
+        for (int w = 1; w < NT / 64; ++w) m = smin(m, red[w]);
+        if (m < bits_to_double(kInfBits)) atomicMin(target, double_to_bits(m));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Directional sweep: sweep_direction_dispatch + solve_line
+// (sweep_core.hpp:16-202, muscl.hpp:197-329, hllc.hpp:62-195).
+//
+// A block owns W lines x L positions along the sweep axis; positions
+// [3, L-3) are written (T = L-6 cells per tile); the 3-cell halo on each side
+// is the reference's stencil radius.  Phases, each followed by an LDS
+// exchange with the neighbouring positions:
+//   1  P^n = cons_to_prim(U^n); P_line = enforce_positive(P^n)
+//   2  face primitives from P_line[i-1..i+1], Hancock half step -> U_half, P_half
+//   3  face primitives from P_half -> time-centred face states and their
+//      primitives (the HLLC inputs)
+//   4  HLLC flux on face i+1/2 and the upwind alpha flux
+//   5  conservative update, repair, non-conservative alpha update, repair
+// Cells 0 and n-1 of a line are first order and unpredicted and the end
+// faces copy their neighbours, as in the reference.
+//
+// MODE 0: sweep qin -> qout.  MODE 1: unsplit accumulation
+// qout += (sweep(qin) - qin), then repair (advance_step.hpp:198-222).
+// FUSE_CFL: reduce next step's CFL candidate over interior cells too (ring
+// cells become copies of interior cells with |u| preserved in the boundary
+// pass, so the interior minimum is the next step's dt).
+// ---------------------------------------------------------------------------
+template<typename Pol, int D, int M, int DIR, int L, int W, int MODE, bool FUSE_CFL>
+__global__ void __launch_bounds__(L* W)
+sweep_kernel(const void* __restrict__ qin, void* __restrict__ qout, Grid g, typename Pol::Eos P,
+             StepCtl* __restrict__ ctl, CflArgs cflargs)
+{
+    using R = typename Pol::R;
+    using St = typename Pol::template St<D, M>;
+    using Pr = typename Pol::template Pr<D, M>;
+    using Fx = typename Pol::template Fx<D, M>;
+    using Cell = typename Pol::template Cell<D, M>;
+    constexpr int S = L * W;
+    constexpr int NV = 2 * M + D;
+    constexpr int NREC = 4 * M + 2 * D + 1;
+    constexpr int T = L - 6;
+    __shared__ R sh[NREC * S];
+
+    const double dt = ctl->dt;
+    const double dxs = g.dx[DIR];
+
+    int pos, lane;
+    if (DIR == 0) {
+        pos = threadIdx.x;
+        lane = threadIdx.y;
+    } else {
+        lane = threadIdx.x;
+        pos = threadIdx.y;
+    }
+    const int slot = (DIR == 0) ? lane * L + pos : pos * W + lane;
+    const int slot_l = (DIR == 0) ? slot - 1 : slot - W;
+    const int slot_r = (DIR == 0) ? slot + 1 : slot + W;
+
+    const int n = g.N[DIR];
+    int idx[3] = {0, 0, 0};
+    bool line_ok;
+    int i;
+    if (DIR == 0) {
+        i = blockIdx.x * T - 3 + pos;
+        idx[1] = blockIdx.y * W + lane;
+        idx[2] = blockIdx.z;
+        line_ok = idx[1] < g.N[1];
+    } else if (DIR == 1) {
+        idx[0] = blockIdx.x * W + lane;
+        i = blockIdx.y * T - 3 + pos;
+        idx[2] = blockIdx.z;
+        line_ok = idx[0] < g.N[0];
+    } else {
+        idx[0] = blockIdx.x * W + lane;
+        idx[1] = blockIdx.z;
+        i = blockIdx.y * T - 3 + pos;
+        line_ok = idx[0] < g.N[0];
+    }
+    idx[DIR] = i;
+    const bool in = line_ok && i >= 0 && i < n;
+    const long long cell = (long long)idx[0] + (long long)idx[1] * g.stride[1] + (long long)idx[2] * g.stride[2];
+    const bool end_cell = (i == 0) || (i == n - 1);
+    const bool own = in && pos >= 3 && pos < L - 3;
+
+    if (!ctl->active) {
+        // a step past t_end: the state passes through unchanged, so the
+        // host-side double-buffer flip stays consistent
+        if (MODE == 0 && own) {
+            Cell U;
+            Pol::template load<D, M>(qin, g.pitch, cell, U);
+            Pol::template store<D, M>(qout, g.pitch, cell, U);
+        }
+        return;
+    }
+
+    // phase 1
+    Cell Uc;
+    St U;
+    Pr Pn, Pl;
+    if (in) {
+        Pol::template load<D, M>(qin, g.pitch, cell, Uc);
+        Pol::template to_state<D, M>(Uc, U);
+        Pol::template cons_to_prim<D, M>(P, U, Pn);
+        Pl = Pn;
+        Pol::template enforce<D, M>(Pl);
+        put_prim<R, D, M>(sh, S, slot, 0, Pl);
+    }
+    __syncthreads();
+
+    // phase 2
+    St Uh;
+    Pr Ph;
+    const bool ph2 = in && pos >= 1 && pos < L - 1;
+    if (ph2) {
+        if (end_cell) {
+            Uh = U;
+            Ph = Pl;
+        } else {
+            Pr PL, PR, lf, rf;
+            get_prim<R, D, M>(sh, S, slot_l, 0, PL);
+            get_prim<R, D, M>(sh, S, slot_r, 0, PR);
+            Pol::template reconstruct<D, M>(PL, Pl, PR, lf, rf);
+            St ul, ur;
+            Pol::template prim_to_cons<D, M>(P, lf, ul);
+            Pol::template prim_to_cons<D, M>(P, rf, ur);
+            Pol::template repair<D, M>(P, ul);
+            Pol::template repair<D, M>(P, ur);
+            Pol::template hancock<D, M, DIR>(P, U, Pl, ul, ur, lf, rf, dt, dxs, Uh);
+            Pol::template cons_to_prim<D, M>(P, Uh, Ph);
+            Pol::template enforce<D, M>(Ph);
+        }
+    }
+    __syncthreads();
+    if (ph2) put_prim<R, D, M>(sh, S, slot, 0, Ph);
+    __syncthreads();
+
+    // phase 3
+    St Ulh, Urh;
+    Pr PUl, PUr;
+    const bool ph3 = in && pos >= 2 && pos < L - 2;
+    if (ph3) {
+        if (end_cell) {
+            Ulh = U;
+            Urh = U;
+            PUl = Pn;
+            PUr = Pn;
+        } else {
+            Pr PL, PR, lf, rf;
+            get_prim<R, D, M>(sh, S, slot_l, 0, PL);
+            get_prim<R, D, M>(sh, S, slot_r, 0, PR);
+            Pol::template reconstruct<D, M>(PL, Ph, PR, lf, rf);
+            Pol::template prim_to_cons<D, M>(P, lf, Ulh);
+            Pol::template prim_to_cons<D, M>(P, rf, Urh);
+            Pol::template repair<D, M>(P, Ulh);
+            Pol::template repair<D, M>(P, Urh);
+            Pol::template cons_to_prim<D, M>(P, Ulh, PUl);
+            Pol::template cons_to_prim<D, M>(P, Urh, PUr);
+        }
+    }
+    __syncthreads();
+    if (ph3) {
+        put_state<R, D, M>(sh, S, slot, 0, Ulh);
+        put_prim<R, D, M>(sh, S, slot, NV, PUl);
+    }
+    __syncthreads();
+
+    // phase 4: face i+1/2
+    Fx F;
+    R fv = R(0);
+    R af[M];
+    const bool ph4 = in && pos >= 2 && pos < L - 3 && i <= n - 2;
+    if (ph4) {
+        St UR;
+        Pr PR;
+        get_state<R, D, M>(sh, S, slot_r, 0, UR);
+        get_prim<R, D, M>(sh, S, slot_r, NV, PR);
+        if (!Pol::template hllc<D, M, DIR>(P, Urh, UR, PUr, PR, F, fv)) atomicOr(&ctl->err, ERR_NONFINITE_HLLC);
+        const R* up = (fv >= R(0)) ? PUr.alpha : PR.alpha;
+#pragma unroll
+        for (int k = 0; k < M; ++k) af[k] = up[k] * fv;
+    }
+    __syncthreads();
+    if (ph4) {
+        int f = 0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) sh[(f++) * S + slot] = F.m[k];
+#pragma unroll
+        for (int d = 0; d < D; ++d) sh[(f++) * S + slot] = F.mom[d];
+        sh[(f++) * S + slot] = F.E;
+        sh[(f++) * S + slot] = fv;
+#pragma unroll
+        for (int k = 0; k < M; ++k) sh[(f++) * S + slot] = af[k];
+    }
+    __syncthreads();
+
+    // phase 5: update cell i
+    double cand = bits_to_double(kInfBits);
+    if (own) {
+        Cell r = Uc;
+        if (n > 1) {
+            Fx Fl, Fr;
+            R fvl, fvr, afl[M], afr[M];
+            if (i <= n - 2) {
+                Fr = F;
+                fvr = fv;
+#pragma unroll
+                for (int k = 0; k < M; ++k) afr[k] = af[k];
+            }
+            if (i >= 1) {
+                int f = 0;
+#pragma unroll
+                for (int k = 0; k < M; ++k) Fl.m[k] = sh[(f++) * S + slot_l];
+#pragma unroll
+                for (int d = 0; d < D; ++d) Fl.mom[d] = sh[(f++) * S + slot_l];
+                Fl.E = sh[(f++) * S + slot_l];
+                fvl = sh[(f++) * S + slot_l];
+#pragma unroll
+                for (int k = 0; k < M; ++k) afl[k] = sh[(f++) * S + slot_l];
+            }
+            if (i == 0) {  // F[0] = F[1]; alpha_flux[0] = alpha_full[0] * fv[0]
+                Fl = Fr;
+                fvl = fvr;
+#pragma unroll
+                for (int k = 0; k < M; ++k) afl[k] = Pn.alpha[k] * fvl;
+            }
+            if (i == n - 1) {  // F[n] = F[n-1]; alpha_flux[n] = alpha_full[n-1] * fv[n]
+                Fr = Fl;
+                fvr = fvl;
+#pragma unroll
+                for (int k = 0; k < M; ++k) afr[k] = Pn.alpha[k] * fvr;
+            }
+            Pol::template update<D, M>(P, Uc, Pn, Fl, Fr, fvl, fvr, afl, afr, dt, dxs, r);
+        }
+        if (MODE == 1) {
+            Cell acc;
+            Pol::template load<D, M>(qout, g.pitch, cell, acc);
+            Pol::template accumulate<D, M>(P, acc, r, Uc);
+            r = acc;
+        }
+        Pol::template store<D, M>(qout, g.pitch, cell, r);
+        if (FUSE_CFL) {
+            bool interior = true;
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+                if (g.N[d] > 1 && (idx[d] == 0 || idx[d] == g.N[d] - 1)) interior = false;
+            if (interior) cand = Pol::template cfl_candidate<D, M>(P, r, cflargs);
+        }
+    }
+    if (FUSE_CFL) block_min_atomic<S>(cand, &ctl->dtmin[ctl->slot ^ 1]);
+}
+
+// ---------------------------------------------------------------------------
+// Ring boundary pass (boundary.hpp:165-226).  Each ring cell is owned by the
+// face of its lowest boundary axis; with extent >= 3 on every axis longer
+// than 1 the source cells are interior, so ring cells are independent.
+// ---------------------------------------------------------------------------
+struct BcArgs {
+    int bc_lo[3];
+    int bc_hi[3];
+    long long face_start[7];  // prefix sums of ring cells per face (d, side) = 2d+side
+};
+
+template<typename Pol, int D, int M>
+__device__ __forceinline__ void nonreflective_state(const typename Pol::Eos& P, const void* q, const Grid& g,
+                                                    const int* idx, const int* src,
+                                                    typename Pol::template Cell<D, M>& out)
+{
+    // local_nonreflective_state (boundary.hpp:150-218)
+    using Cell = typename Pol::template Cell<D, M>;
+    int second[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        second[d] = src[d];
+        if (idx[d] == 0 && g.N[d] > 2) second[d] = 2;
+        else if (idx[d] == g.N[d] - 1 && g.N[d] > 2) second[d] = g.N[d] - 3;
+    }
+    long long sl = 0, tl = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        sl += (long long)src[d] * g.stride[d];
+        tl += (long long)second[d] * g.stride[d];
+    }
+    Cell first, sec, ex;
+    Pol::template load<D, M>(q, g.pitch, sl, first);
+    Pol::template load<D, M>(q, g.pitch, tl, sec);
+    Cell st = first;
+#pragma unroll
+    for (int k = 0; k < M; ++k) ex.m[k] = first.m[k] + (first.m[k] - sec.m[k]);
+#pragma unroll
+    for (int d = 0; d < D; ++d) ex.mom[d] = first.mom[d] + (first.mom[d] - sec.mom[d]);
+    ex.E = first.E + (first.E - sec.E);
+#pragma unroll
+    for (int k = 0; k < M - 1; ++k) ex.a[k] = first.a[k] + (first.a[k] - sec.a[k]);
+    int axis = 0;
+    for (int d = D - 1; d >= 0; --d)
+        if (idx[d] == 0 || idx[d] == g.N[d] - 1) axis = d;
+    const double outward = (idx[axis] == 0) ? -1.0 : 1.0;
+    double ex_rho = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) ex_rho += finite_or((double)ex.m[k]);
+    ex_rho = smax(ex_rho, 1e-12);
+    double exm = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        if (d == axis) exm = ex.mom[d];
+    const double ex_un = outward * safe_div(exm, ex_rho);
+    if (ex_un > 0.0) {
+        double rho = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) rho += finite_or((double)st.m[k]);
+        rho = smax(rho, 1e-12);
+        double m2b = 0.0, m2a = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) m2b += (double)st.mom[d] * (double)st.mom[d];
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            if (d == axis) st.mom[d] = ex.mom[d];
+#pragma unroll
+        for (int d = 0; d < D; ++d) m2a += (double)st.mom[d] * (double)st.mom[d];
+        st.E += 0.5 * (m2a - m2b) / rho;
+    }
+    Pol::template repair_cell<D, M>(P, st);
+    out = st;
+}
+
+template<typename Pol, int D, int M>
+__global__ void __launch_bounds__(256)
+bc_kernel(void* __restrict__ q, Grid g, typename Pol::Eos P, BcArgs bc, StepCtl* __restrict__ ctl, int end_step)
+{
+    using Cell = typename Pol::template Cell<D, M>;
+    const long long total = bc.face_start[2 * D];
+    const bool active = ctl->active;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; active && t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        int face = 0;
+        while (face < 2 * D - 1 && t >= bc.face_start[face + 1]) ++face;
+        long long local = t - bc.face_start[face];
+        const int fd = face >> 1, side = face & 1;
+        int idx[3] = {0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (d == fd) continue;
+            int lo = 0, cnt = g.N[d];
+            if (d < fd && g.N[d] > 1) {
+                lo = 1;
+                cnt = g.N[d] - 2;
+            }
+            idx[d] = lo + (int)(local % cnt);
+            local /= cnt;
+        }
+        idx[fd] = side ? g.N[fd] - 1 : 0;
+        int src[3] = {0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            src[d] = idx[d];
+            if (g.N[d] <= 1) continue;
+            if (idx[d] == 0) src[d] = 1;
+            else if (idx[d] == g.N[d] - 1) src[d] = g.N[d] - 2;
+        }
+        int name = BC_TRANSMISSIVE;
+        for (int d = 0; d < D; ++d) {
+            if (idx[d] == 0) {
+                name = bc.bc_lo[d];
+                break;
+            }
+            if (idx[d] == g.N[d] - 1) {
+                name = bc.bc_hi[d];
+                break;
+            }
+        }
+        long long lin = 0, sl = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            lin += (long long)idx[d] * g.stride[d];
+            sl += (long long)src[d] * g.stride[d];
+        }
+        Cell st;
+        if (name == BC_NONREFLECTIVE) {
+            nonreflective_state<Pol, D, M>(P, q, g, idx, src, st);
+        } else {
+            Pol::template load<D, M>(q, g.pitch, sl, st);
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+                if ((idx[d] == 0 && bc.bc_lo[d] == BC_REFLECTIVE) || (idx[d] == g.N[d] - 1 && bc.bc_hi[d] == BC_REFLECTIVE))
+                    st.mom[d] *= -1;
+        }
+        Pol::template store<D, M>(q, g.pitch, lin, st);
+    }
+    if (end_step && blockIdx.x == 0 && threadIdx.x == 0 && active) {
+        ctl->time += ctl->dt;
+        ctl->step += 1;
+        ctl->slot ^= 1;
+    }
+}
+
+// compute_dt_cfl over every cell into ctl->dtmin[ctl->slot]
+template<typename Pol, int D, int M>
+__global__ void __launch_bounds__(256)
+cfl_kernel(const void* __restrict__ q, Grid g, typename Pol::Eos P, StepCtl* __restrict__ ctl, CflArgs a)
+{
+    using Cell = typename Pol::template Cell<D, M>;
+    double cand = bits_to_double(kInfBits);
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < g.ncells;
+         c += (long long)gridDim.x * blockDim.x) {
+        Cell U;
+        Pol::template load<D, M>(q, g.pitch, c, U);
+        cand = smin(cand, Pol::template cfl_candidate<D, M>(P, U, a));
+    }
+    block_min_atomic<256>(cand, &ctl->dtmin[ctl->slot]);
+}
+
+// packed per-cell doubles <-> policy storage
+template<typename Pol, int D, int M>
+__global__ void pack_in_kernel(typename Pol::Eos P, const double* __restrict__ aos, void* __restrict__ q,
+                               long long first, long long count, long long pitch)
+{
+    using Cell = typename Pol::template Cell<D, M>;
+    constexpr int NV = 2 * M + D;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < count;
+         t += (long long)gridDim.x * blockDim.x) {
+        Cell U;
+        Pol::template from_packed<D, M>(P, aos + t * NV, U);
+        Pol::template store<D, M>(q, pitch, first + t, U);
+    }
+}
+
+template<typename Pol, int D, int M>
+__global__ void pack_out_kernel(typename Pol::Eos P, const void* __restrict__ q, double* __restrict__ aos,
+                                long long first, long long count, long long pitch)
+{
+    using Cell = typename Pol::template Cell<D, M>;
+    constexpr int NV = 2 * M + D;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < count;
+         t += (long long)gridDim.x * blockDim.x) {
+        Cell U;
+        Pol::template load<D, M>(q, pitch, first + t, U);
+        Pol::template to_packed<D, M>(P, U, aos + t * NV);
+    }
+}
+
+// Conservation totals (conservation_report.hpp:188-227) in FP64, fixed order:
+// block b sums cells [b*chunk, (b+1)*chunk) in a fixed tree; a second pass
+// sums the block partials in order.
+template<typename Pol, int D, int M>
+__global__ void __launch_bounds__(256)
+totals_kernel(const void* __restrict__ q, Grid g, typename Pol::Eos P, double volume, long long chunk,
+              double* __restrict__ partial)
+{
+    using Cell = typename Pol::template Cell<D, M>;
+    constexpr int NO = M + 1 + D + 1;
+    __shared__ double red[NO][256];
+    double acc[NO];
+#pragma unroll
+    for (int o = 0; o < NO; ++o) acc[o] = 0.0;
+    const long long beg = (long long)blockIdx.x * chunk;
+    long long end = beg + chunk;
+    if (end > g.ncells) end = g.ncells;
+    for (long long c = beg + threadIdx.x; c < end; c += 256) {
+        Cell U;
+        Pol::template load<D, M>(q, g.pitch, c, U);
+        double cm = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const double part = Pol::template mass<D, M>(U, k) * volume;
+            acc[k] += part;
+            cm += part;
+        }
+        acc[M] += cm;
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc[M + 1 + d] += Pol::template mom<D, M>(U, d) * volume;
+        acc[M + 1 + D] += Pol::template energy<D, M>(P, U) * volume;
+    }
+#pragma unroll
+    for (int o = 0; o < NO; ++o) red[o][threadIdx.x] = acc[o];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+#pragma unroll
+            for (int o = 0; o < NO; ++o) red[o][threadIdx.x] += red[o][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int o = 0; o < NO; ++o) partial[(long long)blockIdx.x * NO + o] = red[o][0];
+}
+
+__global__ void sum_partials_kernel(const double* __restrict__ partial, int nblocks, int no, double* __restrict__ out)
+{
+    const int o = threadIdx.x;
+    if (o >= no) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += partial[(long long)b * no + o];
+    out[o] = s;
+}
+
+}  // namespace dimhip

@@ -1,0 +1,110 @@
+"""FP32-mixed form against the FP64 path (itself bit-identical to the
+reference): normalised L1 error of rho, p and alpha, mass conservation, and
+the stiffened-gas tube where FP64 interface-cell pressure recovery matters.
+
+L1 follows the reference's normalised definition
+dx*sum|a-b| / (dx*sum|b|) (tools/quant/reference_error_metrics.hpp:123-137).
+"""
+import numpy as np
+import pytest
+
+from oracle.pyoracle import Checker, make_params
+from diffuse_vs_gfm_mms_amd import dimhip, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def l1(a, b):
+    return float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
+
+
+def fields(oracle, case, U):
+    """rho, p, alpha_0 of packed cells (oracle cons_to_prim: FP64, bisection)."""
+    p = make_params(case.dim, case.materials)
+    P = oracle.cons_to_prim(p, U)
+    nm = case.nmat
+    return U[:, :nm].sum(1), P[:, -1], P[:, nm]
+
+
+def run(case, precision, nsteps, bc_lo=None, bc_hi=None):
+    with dimhip.Solver(case.dim, case.N, case.dx, case.materials, bc_lo or case.bc_lo, bc_hi or case.bc_hi,
+                       precision=precision) as s:
+        s.upload(case.U)
+        m0 = s.conservation()
+        t, n = s.advance(nsteps, case.cfl, 1.0, 0.0)
+        m1 = s.conservation()
+        return s.download(), t, n, m0, m1
+
+
+@pytest.mark.parametrize("nsteps", [20, 200])
+def test_helium_bubble_l1_vs_fp64(oracle, nsteps):
+    """Helium shock-bubble 512x256: rho and alpha L1 <= 1e-5 and p L1 <= 1e-5
+    against the FP64 path after the same number of steps."""
+    c = W.helium_shock_bubble_2d(512, 256)
+    U64, t64, n64, _, _ = run(c, "fp64", nsteps)
+    U32, t32, n32, _, _ = run(c, "fp32_mixed", nsteps)
+    assert n64 == n32 == nsteps
+    assert abs(t32 - t64) <= 1e-5 * t64
+    r64, p64, a64 = fields(oracle, c, U64)
+    r32, p32, a32 = fields(oracle, c, U32)
+    errs = (l1(r32, r64), l1(p32, p64), l1(a32, a64))
+    print(f"L1 after {nsteps} steps: rho {errs[0]:.3e} p {errs[1]:.3e} alpha {errs[2]:.3e}")
+    assert errs[0] <= 1e-5 and errs[2] <= 1e-5 and errs[1] <= 1e-5, errs
+
+
+def test_closed_box_mass_drift():
+    """Helium bubble at 2 atm in still air, all-reflective box, 40 steps: the
+    waves stay clear of the walls, so the ring boundary pass copies still air
+    onto still air and the scheme's only mass change is the reference's
+    inactive-mass removal (primitives.hpp:392-399).  FP64 partial masses
+    updated from FP32 face fluxes telescope like the FP64 path: the
+    FP32-mixed relative mass drift stays within 1e-12 of the FP64 path's."""
+    rest = (1.29, (0.0, 0.0), 101325.0)
+    c = W.shock_bubble((256, 128), (0.0, 0.0), (0.325, 0.1625), [W.AIR, W.HELIUM], 0, 1,
+                       pre=rest, post=rest, bubble=(0.3, (0.0, 0.0), 2.0e5), centre=(0.1625, 0.08125),
+                       radius=0.02, shock_axis=0, shock_position=1.0, thickness=2 * 0.325 / 256,
+                       bc_lo=("reflective", "reflective"), bc_hi=("reflective", "reflective"))
+    _, _, _, m0_64, m1_64 = run(c, "fp64", 40)
+    U32, _, _, m0_32, m1_32 = run(c, "fp32_mixed", 40)
+    M0 = m0_64[2]
+    drift64 = (m1_64[2] - m0_64[2]) / M0
+    drift32 = (m1_32[2] - m0_32[2]) / M0
+    print(f"relative mass drift: fp64 {drift64:.3e} fp32-mixed {drift32:.3e}")
+    assert abs(m0_32[2] - M0) <= 1e-15 * M0
+    assert abs(drift32 - drift64) <= 1e-12
+    # the walls stayed quiet: ring cells still at rest
+    ring = np.concatenate([U32[:256], U32[-256:]])
+    assert np.max(np.abs(ring[:, 2:4])) < 1e-6
+
+
+def test_water_air_tube_fp32_mixed(oracle):
+    """Stiffened-gas water/air tube (1000 cells, 300 steps).  The FP32-mixed
+    form is accurate for gas-gas mixtures; in nearly pure water expanding at
+    high speed (p ~ 3e4 Pa << gamma p_inf = 2.6e9 Pa, u ~ 480 m/s) the shifted
+    internal energy and the kinetic energy cancel to 1e-3 and FP32 energy
+    storage leaves ~1e-3 relative pressure error there; the interface then
+    evolves measurably differently (FP64 itself moves by 4e-5 under 1e-7 input
+    noise).  Measured: rho 2.8e-3, p 2.8e-3, alpha 3.3e-3 normalised L1; the
+    test bounds them at 1e-2 and the FP64 form is the one for stiff liquids."""
+    c = W.water_air_tube_1d(1000)
+    U64, _, _, _, _ = run(c, "fp64", 300)
+    U32, _, _, _, _ = run(c, "fp32_mixed", 300)
+    r64, p64, a64 = fields(oracle, c, U64)
+    r32, p32, a32 = fields(oracle, c, U32)
+    errs = (l1(r32, r64), l1(p32, p64), l1(a32, a64))
+    print(f"water-air L1: rho {errs[0]:.3e} p {errs[1]:.3e} alpha {errs[2]:.3e}")
+    assert max(errs) <= 1e-2, errs
+
+
+def test_fp32_mixed_3d_and_unsplit_run():
+    """3D and unsplit FP32-mixed runs stay finite and close to FP64."""
+    c = W.bubble_collapse_3d(24)
+    for tu in ("split", "unsplit"):
+        outs = []
+        for prec in ("fp64", "fp32_mixed"):
+            with dimhip.Solver(3, c.N, c.dx, c.materials, c.bc_lo, c.bc_hi, time_update=tu, precision=prec) as s:
+                s.upload(c.U)
+                s.advance(10, c.cfl, 1.0)
+                outs.append(s.download())
+        assert np.all(np.isfinite(outs[1]))
+        assert l1(outs[1][:, :2].sum(1), outs[0][:, :2].sum(1)) < 1e-4
