@@ -43,6 +43,8 @@ def parse():
                     help="steps of the FP32-mixed 4096x2048 companion measurement (0: skip)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="N",
+                    help="only run a device copy of N doubles (calibrates HBM byte counters) and exit")
     return ap.parse_args()
 
 
@@ -111,6 +113,14 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     from diffuse_vs_gfm_mms_amd import dimhip, workloads
+
+    if args.copy_probe:
+        import numpy as np
+        src = np.arange(args.copy_probe, dtype=np.float64)
+        out, _ = dimhip.kat(dimhip.KAT_COPY, 1, [("ideal_gas", 1.4, 0.0)], args.copy_probe, src)
+        print(json.dumps({"copy_probe_doubles": args.copy_probe, "bytes_read": 8 * args.copy_probe,
+                          "bytes_written": 8 * args.copy_probe, "ok": bool(np.array_equal(out, src))}))
+        return
 
     grid, precision, _ = WORKLOADS[args.workload]
     main_run = measure(dimhip, workloads, grid, precision, args.steps, args.warmup)

@@ -149,9 +149,10 @@ __device__ __forceinline__ double mix_energy(const EosParams& P, const MixCoef<M
     return finite_or(s);
 }
 
-// eos.hpp:77-143 — bracket doubling + bisection.  Every evaluated midpoint
-// and comparison is the reference's; only loop-invariant subexpressions are
-// hoisted (mix_coef), which does not change any rounded value.
+// eos.hpp:77-143 — bracket doubling + bisection, run as the reference runs
+// it; only loop-invariant subexpressions are hoisted (mix_coef), which leaves
+// every rounded value unchanged.  (A replica that skips residuals far from the
+// root measured slower on gfx950: the loop is latency-bound, not residual-bound.)
 template<int M>
 __device__ double pressure_from_internal_energy(const EosParams& P, double rho_total, double e,
                                                 const double* alpha, const double* rho)
@@ -569,7 +570,7 @@ template<int D, int M, int DIR>
 __device__ __forceinline__ void hancock_predict(const EosParams& P, const State<D, M>& Uc,
                                                 const Prim<D, M>& Pc, const State<D, M>& ULf,
                                                 const State<D, M>& URf, const Prim<D, M>& PLf,
-                                                const Prim<D, M>& PRf, double dt, double dx,
+                                                const Prim<D, M>& PRf, const double* rhs, double dt, double dx,
                                                 State<D, M>& out)
 {
     Flux<D, M> FL, FR;
@@ -585,8 +586,7 @@ __device__ __forceinline__ void hancock_predict(const EosParams& P, const State<
     const double u_left = PLf.vel[DIR];
     const double u_right = PRf.vel[DIR];
     const double div_u = (u_right - u_left) / dx;
-    double rhs[M];
-    alpha_rhs_coefficients<D, M>(P, Pc, rhs);
+    // rhs = alpha_rhs_coefficients(Pc), computed by the caller
     double an[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) {

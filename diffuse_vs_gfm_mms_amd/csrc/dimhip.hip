@@ -339,6 +339,13 @@ __global__ void copy_kernel(const double* __restrict__ a, double* __restrict__ b
         b[t] = a[t];
 }
 
+// a plain streaming copy of known size: the calibration kernel for HBM byte counters
+__global__ void copy_probe_kernel(const double* __restrict__ a, double* __restrict__ b, long long n)
+{
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        b[t] = a[t];
+}
+
 void* state_ptr(dimhip_ctx* c, int which) { return c->q[which]; }
 
 void copy_state(dimhip_ctx* c, const void* a, void* b)
@@ -478,6 +485,9 @@ int dimhip_create(const dimhip_desc* desc, dimhip_ctx** out)
     c->eos.lambda_model = desc->lambda_model;
     c->eos.alpha_source_floor = desc->alpha_source_floor;
     c->eosm.lambda_model = desc->lambda_model;
+    c->eosm.stiff = 0;
+    for (int k = 0; k < desc->nmat; ++k)
+        if (desc->eos_kind[k] == DIMHIP_EOS_STIFFENED) c->eosm.stiff = 1;
     c->eosm.alpha_source_floor = (float)desc->alpha_source_floor;
     // ring cells per face (d, side), owned by the face of their lowest boundary axis
     long long acc = 0;
@@ -776,6 +786,8 @@ int dimhip_kat(const dimhip_desc* desc, int which, int dir, int count, const dou
         case 3: n_in0 = n_in1 = (size_t)count * nv; n_out0 = (size_t)count * nf; n_out1 = count; break;
         case 4: n_in0 = (size_t)count * np; n_out0 = (size_t)count * M; break;
         case 5: n_in0 = (size_t)count * 2; n_out0 = n_out1 = count; break;
+        case 6: n_in0 = (size_t)count * (2 + 2 * M); n_out0 = n_out1 = count; break;
+        case 7: n_in0 = n_out0 = (size_t)count; break;
         default: return fail(nullptr, "dimhip_kat: unknown function");
     }
     if (which == 3 && (dir < 0 || dir >= D)) return fail(nullptr, "dimhip_kat: direction out of range");
@@ -812,6 +824,8 @@ int dimhip_kat(const dimhip_desc* desc, int which, int dir, int count, const dou
     int rc = 0;
     if (which == 5) {
         kat_ieee_kernel<<<(count + 127) / 128, 128>>>(count, di0, do0, do1);
+    } else if (which == 7) {
+        copy_probe_kernel<<<grid_blocks(count, 1024), 256>>>(di0, do0, count);
     } else {
         const Ops* ops = select_ops(D, M);
         rc = ops ? ops->kat(P, which, dir, count, di0, di1, do0, do1, derr, nullptr) : 1;

@@ -147,6 +147,11 @@ __global__ void kat64_kernel(EosParams P, int which, int count, const double* __
         for (int d = 0; d < D; ++d) o[j++] = F.mom[d];
         o[j++] = F.E;
         out1[t] = fv;
+    } else if (which == 6) {
+        // [rho_total, e, alpha[M], rho[M]] -> pressure (out0 and out1)
+        const double* r = in0 + (long long)t * (2 + 2 * M);
+        out0[t] = pressure_from_internal_energy<M>(P, r[0], r[1], r + 2, r + 2 + M);
+        out1[t] = out0[t];
     } else if (which == 4) {
         Prim<D, M> Q;
         load_packed_prim<D, M>(in0 + (long long)t * LY::NP, Q);

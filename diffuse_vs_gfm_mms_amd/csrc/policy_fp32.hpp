@@ -32,6 +32,7 @@ struct EosMixed {
     double b_d[kMaxMat];
     int lambda_model;
     float alpha_source_floor;
+    int stiff;                // any stiffened-gas material: B(alpha) is not identically zero
 };
 
 namespace f32 {
@@ -152,6 +153,7 @@ __device__ __forceinline__ void encode_alpha(const float* alpha, float* a)
 template<int M>
 __device__ __forceinline__ float bshift(const EosMixed& P, const float* from, const float* to)
 {
+    if (!P.stiff) return 0.f;
     float s = 0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) s += P.b[k] * (from[k] - to[k]);
@@ -161,6 +163,7 @@ __device__ __forceinline__ float bshift(const EosMixed& P, const float* from, co
 template<int M>
 __device__ __forceinline__ float bdot(const EosMixed& P, const float* alpha)
 {
+    if (!P.stiff) return 0.f;
     float s = 0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) s += P.b[k] * alpha[k];
@@ -232,11 +235,13 @@ __device__ __forceinline__ void cons_to_prim(const EosMixed& P, const St<D, M>& 
     const float shift = bshift<M>(P, ad, R.alpha);
     float p;
     if (interface_cell<M>(R.alpha)) {
+        // FP64 recovery: the energy minus kinetic energy difference, where the
+        // cancellation sits, in FP64 (float products are exact in FP64)
         double m2 = 0.0;
 #pragma unroll
         for (int d = 0; d < D; ++d) m2 += (double)ffinite_or(U.mom[d]) * (double)ffinite_or(U.mom[d]);
         const double rho_e_hat = (double)ffinite_or(U.E, kinetic) + (double)shift - 0.5 * m2 / (double)rho;
-        p = pressure_shifted_d<M>(P, rho_e_hat, R.alpha);
+        p = pressure_shifted<M>(P, (float)rho_e_hat, R.alpha);
     } else {
         p = pressure_shifted<M>(P, ffinite_or(U.E, kinetic) + shift - kinetic, R.alpha);
     }
@@ -508,8 +513,8 @@ __device__ __forceinline__ void flux_normal(const EosMixed& P, const St<D, M>& U
 // muscl.hpp:143-191
 template<int D, int M, int DIR>
 __device__ __forceinline__ void hancock(const EosMixed& P, const St<D, M>& Uc, const Pr<D, M>& Pc, const St<D, M>& ul,
-                                        const St<D, M>& ur, const Pr<D, M>& lf, const Pr<D, M>& rf, float dt, float dx,
-                                        St<D, M>& out)
+                                        const St<D, M>& ur, const Pr<D, M>& lf, const Pr<D, M>& rf, const float* rhs,
+                                        float dt, float dx, St<D, M>& out)
 {
     Fx<D, M> FL, FR;
     flux_normal<D, M, DIR>(P, ul, lf, FL);
@@ -524,8 +529,7 @@ __device__ __forceinline__ void hancock(const EosMixed& P, const St<D, M>& Uc, c
     out.E -= factor * (FR.E - FL.E);
     const float ul_ = lf.vel[DIR], ur_ = rf.vel[DIR];
     const float div_u = (ur_ - ul_) * inv_dx;
-    float rhs[M], an[M];
-    alpha_rhs<D, M>(P, Pc, rhs);
+    float an[M];
 #pragma unroll
     for (int k = 0; k < M; ++k)
         an[k] = Pc.alpha[k] - factor * (rf.alpha[k] * ur_ - lf.alpha[k] * ul_) + 0.5f * dt * rhs[k] * div_u;
@@ -763,6 +767,24 @@ struct Fp32Mixed {
     }
     template<int D, int M>
     static __device__ __forceinline__ void enforce(Pr<D, M>& Q) { f32::enforce<D, M>(Q); }
+    // after cons_to_prim the densities are floored and the fractions
+    // sanitised already; only the pressure floor of enforce_positive applies
+    template<int D, int M>
+    static __device__ __forceinline__ void enforce_c2p(Pr<D, M>& Q) { Q.p = f32::fmaxs(Q.p, 1e-12f); }
+
+    template<int D, int M>
+    static __device__ __forceinline__ void alpha_rhs(const Eos& P, const Pr<D, M>& Pn, float* rhs)
+    {
+        f32::alpha_rhs<D, M>(P, Pn, rhs);
+    }
+    // P_line differs from P^n only by a second renormalisation of the
+    // fractions: the predictor reuses the coefficients of P^n
+    template<int D, int M>
+    static __device__ __forceinline__ void alpha_rhs_line(const Eos&, const Pr<D, M>&, const float* rhs_n, float* rhs)
+    {
+#pragma unroll
+        for (int k = 0; k < M; ++k) rhs[k] = rhs_n[k];
+    }
     template<int D, int M>
     static __device__ __forceinline__ void reconstruct(const Pr<D, M>& L, const Pr<D, M>& C, const Pr<D, M>& Rt,
                                                        Pr<D, M>& lf, Pr<D, M>& rf)
@@ -776,12 +798,61 @@ struct Fp32Mixed {
     }
     template<int D, int M>
     static __device__ __forceinline__ void repair(const Eos& P, St<D, M>& U) { f32::repair<D, M>(P, U); }
+
+    // repair_state(prim_to_cons(Q)) changes nothing unless a fraction at or
+    // below the inactive threshold (1e-10) carries more than 1e-12 mass: masses
+    // are non-negative, fractions sanitised and the energy above its floor by
+    // construction.  Only then is the full repair run.
+    template<int D, int M>
+    static __device__ __forceinline__ bool needs_repair(const St<D, M>& U)
+    {
+        float ad[M];
+        f32::decode_alpha<M>(U.a, ad);
+        bool r = false;
+#pragma unroll
+        for (int k = 0; k < M; ++k) r |= (ad[k] <= 1e-10f && U.m[k] > 1e-12f);
+        return r;
+    }
+
+    template<int D, int M>
+    static __device__ __forceinline__ void face_cons(const Eos& P, const Pr<D, M>& Q, St<D, M>& U)
+    {
+        f32::prim_to_cons<D, M>(P, Q, U);
+        if (needs_repair<D, M>(U)) f32::repair<D, M>(P, U);
+    }
+
+    // face state and its primitive: when repair has nothing to do the
+    // primitive of prim_to_cons(Q) is Q with the same floors, so the
+    // pressure round trip is skipped
+    template<int D, int M>
+    static __device__ __forceinline__ void face_state(const Eos& P, const Pr<D, M>& Q, St<D, M>& U, Pr<D, M>& QU)
+    {
+        f32::prim_to_cons<D, M>(P, Q, U);
+        if (needs_repair<D, M>(U)) {
+            f32::repair<D, M>(P, U);
+            f32::cons_to_prim<D, M>(P, U, QU);
+            return;
+        }
+        float alpha[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) alpha[k] = Q.alpha[k];
+        f32::sanitise<M>(alpha, 1e-12f);
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            QU.alpha[k] = alpha[k];
+            QU.rho[k] = f32::fmaxs(Q.rho[k], 1e-12f);
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) QU.vel[d] = Q.vel[d];
+        QU.p = f32::fmaxs(Q.p, 1e-10f);
+    }
     template<int D, int M, int DIR>
     static __device__ __forceinline__ void hancock(const Eos& P, const St<D, M>& Uc, const Pr<D, M>& Pc,
                                                    const St<D, M>& ul, const St<D, M>& ur, const Pr<D, M>& lf,
-                                                   const Pr<D, M>& rf, double dt, double dx, St<D, M>& out)
+                                                   const Pr<D, M>& rf, const float* rhs, double dt, double dx,
+                                                   St<D, M>& out)
     {
-        f32::hancock<D, M, DIR>(P, Uc, Pc, ul, ur, lf, rf, (float)dt, (float)dx, out);
+        f32::hancock<D, M, DIR>(P, Uc, Pc, ul, ur, lf, rf, rhs, (float)dt, (float)dx, out);
     }
     template<int D, int M, int DIR>
     static __device__ __forceinline__ bool hllc(const Eos& P, const St<D, M>& UL, const St<D, M>& UR,
@@ -792,7 +863,8 @@ struct Fp32Mixed {
 
     template<int D, int M>
     static __device__ __forceinline__ void update(const Eos& P, const Cell<D, M>& Uin, const Pr<D, M>& Pn,
-                                                  const Fx<D, M>& Fl, const Fx<D, M>& Fr, float fvl, float fvr,
+                                                  const float* rhs, const Fx<D, M>& Fl, const Fx<D, M>& Fr,
+                                                  float fvl, float fvr,
                                                   const float* afl, const float* afr, double dt, double dx,
                                                   Cell<D, M>& r)
     {
@@ -805,8 +877,7 @@ struct Fp32Mixed {
         for (int d = 0; d < D; ++d) r.mom[d] -= lf * (Fr.mom[d] - Fl.mom[d]);
         r.E -= lf * (Fr.E - Fl.E);
         f32::repair<D, M>(P, r);
-        float rhs[M], an[M];
-        f32::alpha_rhs<D, M>(P, Pn, rhs);
+        float an[M];
         const float div_u = (fvr - fvl) * (float)(1.0 / dx);
         const float dtf = (float)dt;
 #pragma unroll

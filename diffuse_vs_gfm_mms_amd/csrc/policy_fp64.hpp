@@ -92,6 +92,23 @@ struct Fp64 {
     }
     template<int D, int M>
     static __device__ __forceinline__ void enforce(Pr<D, M>& Q) { enforce_positive_primitive<D, M>(Q); }
+    // enforce_positive_primitive after cons_to_prim (muscl.hpp:231-232, 294-295)
+    template<int D, int M>
+    static __device__ __forceinline__ void enforce_c2p(Pr<D, M>& Q) { enforce_positive_primitive<D, M>(Q); }
+
+    // alpha_rhs_coefficients(P^n) (sweep_core.hpp:131-137)
+    template<int D, int M>
+    static __device__ __forceinline__ void alpha_rhs(const Eos& P, const Pr<D, M>& Pn, double* rhs)
+    {
+        alpha_rhs_coefficients<D, M>(P, Pn, rhs);
+    }
+    // the predictor's coefficients come from P_line (muscl.hpp:173-179)
+    template<int D, int M>
+    static __device__ __forceinline__ void alpha_rhs_line(const Eos& P, const Pr<D, M>& Pl, const double*,
+                                                          double* rhs)
+    {
+        alpha_rhs_coefficients<D, M>(P, Pl, rhs);
+    }
     template<int D, int M>
     static __device__ __forceinline__ void reconstruct(const Pr<D, M>& L, const Pr<D, M>& C, const Pr<D, M>& Rt,
                                                        Pr<D, M>& lf, Pr<D, M>& rf)
@@ -105,12 +122,31 @@ struct Fp64 {
     }
     template<int D, int M>
     static __device__ __forceinline__ void repair(const Eos& P, St<D, M>& U) { repair_state<D, M>(P, U); }
+
+    // reconstructed face primitive -> face state, as the reference builds it
+    // (muscl.hpp:263-266, 319-322): prim_to_cons then repair_state
+    template<int D, int M>
+    static __device__ __forceinline__ void face_cons(const Eos& P, const Pr<D, M>& Q, St<D, M>& U)
+    {
+        dimhip::prim_to_cons<D, M>(P, Q, U);
+        repair_state<D, M>(P, U);
+    }
+
+    // ... and its primitive for the Riemann solver (sweep_core.hpp:78-81)
+    template<int D, int M>
+    static __device__ __forceinline__ void face_state(const Eos& P, const Pr<D, M>& Q, St<D, M>& U, Pr<D, M>& QU)
+    {
+        dimhip::prim_to_cons<D, M>(P, Q, U);
+        repair_state<D, M>(P, U);
+        dimhip::cons_to_prim<D, M>(P, U, QU);
+    }
     template<int D, int M, int DIR>
     static __device__ __forceinline__ void hancock(const Eos& P, const St<D, M>& Uc, const Pr<D, M>& Pc,
                                                    const St<D, M>& ul, const St<D, M>& ur, const Pr<D, M>& lf,
-                                                   const Pr<D, M>& rf, double dt, double dx, St<D, M>& out)
+                                                   const Pr<D, M>& rf, const double* rhs, double dt, double dx,
+                                                   St<D, M>& out)
     {
-        hancock_predict<D, M, DIR>(P, Uc, Pc, ul, ur, lf, rf, dt, dx, out);
+        hancock_predict<D, M, DIR>(P, Uc, Pc, ul, ur, lf, rf, rhs, dt, dx, out);
     }
     template<int D, int M, int DIR>
     static __device__ __forceinline__ bool hllc(const Eos& P, const St<D, M>& UL, const St<D, M>& UR,
@@ -122,7 +158,8 @@ struct Fp64 {
     // solve_line's update of one cell (sweep_core.hpp:101-148)
     template<int D, int M>
     static __device__ __forceinline__ void update(const Eos& P, const Cell<D, M>& Uin, const Pr<D, M>& Pn,
-                                                  const Fx<D, M>& Fl, const Fx<D, M>& Fr, double fvl, double fvr,
+                                                  const double* rhs, const Fx<D, M>& Fl, const Fx<D, M>& Fr,
+                                                  double fvl, double fvr,
                                                   const double* afl, const double* afr, double dt, double dx,
                                                   Cell<D, M>& r)
     {
@@ -134,8 +171,7 @@ struct Fp64 {
         for (int d = 0; d < D; ++d) r.mom[d] -= lambda * (Fr.mom[d] - Fl.mom[d]);
         r.E -= lambda * (Fr.E - Fl.E);
         repair_state<D, M>(P, r);
-        double rhs[M], an[M];
-        alpha_rhs_coefficients<D, M>(P, Pn, rhs);
+        double an[M];
         const double div_u = (fvr - fvl) / dx;
 #pragma unroll
         for (int k = 0; k < M; ++k) {

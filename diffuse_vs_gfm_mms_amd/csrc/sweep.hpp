@@ -84,6 +84,23 @@ __device__ __forceinline__ void block_min_atomic(double v, unsigned long long* t
     }
 }
 
+// MI355X dispatches workgroups round-robin over its 8 XCDs, each with its own
+// L2.  Renumber blocks so that consecutive logical blocks (neighbouring tiles,
+// which share cache lines and halo cells) run on the same XCD:
+// physical block p is the (p/8)-th block of XCD p%8 and gets logical index
+// start(p%8) + p/8, a bijection for any grid size.
+__device__ __forceinline__ void xcd_block(int& bx, int& by, int& bz)
+{
+    const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+    const unsigned total = gx * gy * gz;
+    const unsigned p = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned x = p & 7u, q = total >> 3, r = total & 7u;
+    const unsigned l = x * q + (x < r ? x : r) + (p >> 3);
+    bx = (int)(l % gx);
+    by = (int)((l / gx) % gy);
+    bz = (int)(l / (gx * gy));
+}
+
 // ---------------------------------------------------------------------------
 // Directional sweep: sweep_direction_dispatch + solve_line
 // (sweep_core.hpp:16-202, muscl.hpp:197-329, hllc.hpp:62-195).
@@ -139,23 +156,25 @@ sweep_kernel(const void* __restrict__ qin, void* __restrict__ qout, Grid g, type
     const int slot_r = (DIR == 0) ? slot + 1 : slot + W;
 
     const int n = g.N[DIR];
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
     int idx[3] = {0, 0, 0};
     bool line_ok;
     int i;
     if (DIR == 0) {
-        i = blockIdx.x * T - 3 + pos;
-        idx[1] = blockIdx.y * W + lane;
-        idx[2] = blockIdx.z;
+        i = bx * T - 3 + pos;
+        idx[1] = by * W + lane;
+        idx[2] = bz;
         line_ok = idx[1] < g.N[1];
     } else if (DIR == 1) {
-        idx[0] = blockIdx.x * W + lane;
-        i = blockIdx.y * T - 3 + pos;
-        idx[2] = blockIdx.z;
+        idx[0] = bx * W + lane;
+        i = by * T - 3 + pos;
+        idx[2] = bz;
         line_ok = idx[0] < g.N[0];
     } else {
-        idx[0] = blockIdx.x * W + lane;
-        idx[1] = blockIdx.z;
-        i = blockIdx.y * T - 3 + pos;
+        idx[0] = bx * W + lane;
+        idx[1] = bz;
+        i = by * T - 3 + pos;
         line_ok = idx[0] < g.N[0];
     }
     idx[DIR] = i;
@@ -179,12 +198,14 @@ sweep_kernel(const void* __restrict__ qin, void* __restrict__ qout, Grid g, type
     Cell Uc;
     St U;
     Pr Pn, Pl;
+    R rhs_n[M];  // alpha_rhs_coefficients(P^n), used by the update
     if (in) {
         Pol::template load<D, M>(qin, g.pitch, cell, Uc);
         Pol::template to_state<D, M>(Uc, U);
         Pol::template cons_to_prim<D, M>(P, U, Pn);
         Pl = Pn;
-        Pol::template enforce<D, M>(Pl);
+        Pol::template enforce_c2p<D, M>(Pl);
+        Pol::template alpha_rhs<D, M>(P, Pn, rhs_n);
         put_prim<R, D, M>(sh, S, slot, 0, Pl);
     }
     __syncthreads();
@@ -203,13 +224,13 @@ sweep_kernel(const void* __restrict__ qin, void* __restrict__ qout, Grid g, type
             get_prim<R, D, M>(sh, S, slot_r, 0, PR);
             Pol::template reconstruct<D, M>(PL, Pl, PR, lf, rf);
             St ul, ur;
-            Pol::template prim_to_cons<D, M>(P, lf, ul);
-            Pol::template prim_to_cons<D, M>(P, rf, ur);
-            Pol::template repair<D, M>(P, ul);
-            Pol::template repair<D, M>(P, ur);
-            Pol::template hancock<D, M, DIR>(P, U, Pl, ul, ur, lf, rf, dt, dxs, Uh);
+            Pol::template face_cons<D, M>(P, lf, ul);
+            Pol::template face_cons<D, M>(P, rf, ur);
+            R rhs_l[M];
+            Pol::template alpha_rhs_line<D, M>(P, Pl, rhs_n, rhs_l);
+            Pol::template hancock<D, M, DIR>(P, U, Pl, ul, ur, lf, rf, rhs_l, dt, dxs, Uh);
             Pol::template cons_to_prim<D, M>(P, Uh, Ph);
-            Pol::template enforce<D, M>(Ph);
+            Pol::template enforce_c2p<D, M>(Ph);
         }
     }
     __syncthreads();
@@ -231,12 +252,8 @@ sweep_kernel(const void* __restrict__ qin, void* __restrict__ qout, Grid g, type
             get_prim<R, D, M>(sh, S, slot_l, 0, PL);
             get_prim<R, D, M>(sh, S, slot_r, 0, PR);
             Pol::template reconstruct<D, M>(PL, Ph, PR, lf, rf);
-            Pol::template prim_to_cons<D, M>(P, lf, Ulh);
-            Pol::template prim_to_cons<D, M>(P, rf, Urh);
-            Pol::template repair<D, M>(P, Ulh);
-            Pol::template repair<D, M>(P, Urh);
-            Pol::template cons_to_prim<D, M>(P, Ulh, PUl);
-            Pol::template cons_to_prim<D, M>(P, Urh, PUr);
+            Pol::template face_state<D, M>(P, lf, Ulh, PUl);
+            Pol::template face_state<D, M>(P, rf, Urh, PUr);
         }
     }
     __syncthreads();
@@ -311,7 +328,7 @@ sweep_kernel(const void* __restrict__ qin, void* __restrict__ qout, Grid g, type
 #pragma unroll
                 for (int k = 0; k < M; ++k) afr[k] = Pn.alpha[k] * fvr;
             }
-            Pol::template update<D, M>(P, Uc, Pn, Fl, Fr, fvl, fvr, afl, afr, dt, dxs, r);
+            Pol::template update<D, M>(P, Uc, Pn, rhs_n, Fl, Fr, fvl, fvr, afl, afr, dt, dxs, r);
         }
         if (MODE == 1) {
             Cell acc;

@@ -26,7 +26,7 @@ LAMBDA = {"kapila": 0, "": 0, "allaire": 1, "equal_velocity": 2}
 TIME_UPDATE = {"split": 0, "unsplit": 1}
 PRECISION = {"fp64": 0, "fp32_mixed": 1}
 
-KAT_CONS_TO_PRIM, KAT_PRIM_TO_CONS, KAT_REPAIR, KAT_HLLC, KAT_ALPHA_RHS, KAT_IEEE = range(6)
+KAT_CONS_TO_PRIM, KAT_PRIM_TO_CONS, KAT_REPAIR, KAT_HLLC, KAT_ALPHA_RHS, KAT_IEEE, KAT_PRESSURE, KAT_COPY = range(8)
 
 
 class Desc(C.Structure):
@@ -234,7 +234,7 @@ def kat(which, dim, materials, count, in0, in1=None, d=0, lambda_model="kapila",
     nv, np_, nf = 2 * nmat + dim, 2 * nmat + dim + 1, nmat + dim + 1
     in0 = np.ascontiguousarray(in0, dtype=np.float64)
     in1 = np.ascontiguousarray(in1 if in1 is not None else np.zeros(1), dtype=np.float64)
-    size0 = {0: np_, 1: nv, 2: nv, 3: nf, 4: nmat, 5: 1}[which] * count
+    size0 = {0: np_, 1: nv, 2: nv, 3: nf, 4: nmat, 5: 1, 6: 1, 7: 1}[which] * count
     out0 = np.zeros(max(size0, 1))
     out1 = np.zeros(max(count, 1))
     rc = lib.dimhip_kat(C.byref(desc), which, d, count, _dp(in0), _dp(in1), _dp(out0), _dp(out1))

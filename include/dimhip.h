@@ -144,6 +144,11 @@ int dimhip_kernel_times(dimhip_ctx* ctx, double* mean_ms, long long* counts);
  *                    -> out0: count*(nmat+dim+1) fluxes, out1: count face velocities
  *   4 alpha_rhs_coefficients (primitives.hpp:216-322) in0: primitives -> out0: count*nmat
  *   5 IEEE check: out0[i] = in0[2i] / in0[2i+1], out1[i] = sqrt(in0[2i]) in FP64
+ *   6 MixtureEOS::pressure_from_internal_energy (src/dim/eos.hpp:77-143) on
+ *     in0: count*(2+2nmat) = [rho_total, e, alpha[nmat], rho[nmat]] ->
+ *     out0 (and out1): the pressure
+ *   7 device copy of count doubles in0 -> out0 (a streaming kernel of known
+ *     size, used to calibrate HBM byte counters)
  * Host buffers; the call allocates and frees its own device memory. */
 int dimhip_kat(const dimhip_desc* desc, int which, int dir, int count, const double* in0,
                const double* in1, double* out0, double* out1);

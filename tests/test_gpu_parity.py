@@ -245,3 +245,33 @@ def test_helium_2048x1024_step_matches_reference():
     assert bits_equal(gdts, dts)
     assert np.all(rel_linf(out, U) <= 1e-12)
     assert bits_equal(out, U), mismatch_report(out, U)
+
+
+@pytest.mark.parametrize("ms", ["water_air", "air_helium", "water_air_helium", "air"])
+def test_pressure_bisection_matches_oracle(oracle, ms):
+    """The device pressure bisection equals the oracle's, bit for bit, over
+    wide-range and adversarial inputs: tiny/huge/negative energies, vanishing
+    fractions, floored densities, non-finite values."""
+    mats = MATSETS[ms]
+    nm = len(mats)
+    p = make_params(1, mats)
+    rng = np.random.default_rng(2024)
+    n = 60000
+    rt = 10.0 ** rng.uniform(-13, 4, n)
+    e = 10.0 ** rng.uniform(-14, 9, n) * rng.choice([1.0, 1.0, 1.0, -1.0], n)
+    alpha = rng.uniform(0, 1, (n, nm))
+    alpha[rng.random(n) < 0.2, 0] = 0.0
+    alpha[rng.random(n) < 0.1, -1] = 1e-13
+    alpha /= alpha.sum(1, keepdims=True)
+    rho = 10.0 ** rng.uniform(-13, 4, (n, nm))
+    special = rng.integers(0, 40, n)
+    e[special == 0] = 0.0
+    e[special == 1] = np.nan
+    rho[special == 2, 0] = np.inf
+    alpha[special == 3, 0] = np.nan
+    e[special == 4] = 1e300
+    rt[special == 5] = -1.0
+    packed = np.concatenate([rt[:, None], e[:, None], alpha, rho], axis=1)
+    got, _ = dimhip.kat(dimhip.KAT_PRESSURE, 1, mats, n, packed)
+    ref = oracle.pressure(p, rt, e, alpha, rho)
+    assert bits_equal(got, ref), mismatch_report(got, ref)
