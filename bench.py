@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--workload", default="helium_fp64", choices=sorted(WORKLOADS))
     ap.add_argument("--fp32-steps", type=int, default=2000,
                     help="steps of the FP32-mixed 4096x2048 companion measurement (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=1000,
+                    help="steps of the batched 64-instance companion measurement (0: skip)")
+    ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="N",
@@ -104,6 +107,41 @@ def measure(dimhip, workloads, grid, precision, steps, warmup):
     if n != steps:
         raise RuntimeError(f"took {n} steps, expected {steps}")
     return {"case": case, "value": case.ncells * steps / wall, "wall": wall, "ms": ms, "counts": counts}
+
+
+def measure_batch(dimhip, workloads, count, steps):
+    """Seeded uneven shock-interface instances, FP32-mixed, advanced together
+    (autotuned tiles and schedule, hipGraph-captured steps)."""
+    cases = workloads.batch_instances(count=count)
+    cells = sum(c.ncells for c in cases)
+    with dimhip.Batch(cases) as b:
+        tune = b.autotune(0.8, trial_steps=3)
+        b.advance(10, 0.8, use_graph=True)
+        t0 = time.perf_counter()
+        _, nsteps = b.advance(steps, 0.8, use_graph=True)
+        wall = time.perf_counter() - t0
+        b.set_timing(True)
+        b.advance(20, 0.8, use_graph=False)
+        ms, counts = b.kernel_times()
+        b.set_timing(False)
+    if min(nsteps) != steps or max(nsteps) != steps:
+        raise RuntimeError("batched instances did not all take the requested steps")
+    per_cell = 2 * (8 * 2 + 4 * 4)  # FP64 masses + FP32 momentum, energy, fraction, read and written
+    sweep_bytes_avg = per_cell * cells
+    achieved = sweep_bytes_avg / (ms[0] * 1e-3) / 1e9
+    return {
+        "workload": f"{count} seeded shock-interface instances, 256..1024 x 256..512, FP32-mixed, "
+                    f"{steps} steps each, one launch per stage, hipGraph-captured steps",
+        "value": cells * steps / wall,
+        "unit": "cell-updates/s",
+        "cells": cells,
+        "ms_per_step": wall / steps * 1e3,
+        "autotuned": tune["best"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "mean_ms": {"sweep": ms[0], "boundary": ms[1], "cfl": ms[2]},
+                     "bytes_per_launch": sweep_bytes_avg},
+    }
 
 
 def main():
@@ -184,6 +222,8 @@ def main():
                          "mean_ms": {"sweep": r32["ms"][0], "boundary": r32["ms"][1], "cfl": r32["ms"][2]},
                          "bytes_per_launch": b32},
         }
+    if args.batch_steps > 0 and precision == "fp64":
+        out["batched"] = measure_batch(dimhip, workloads, args.batch_count, args.batch_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(workloads.helium_shock_bubble_2d(2048, 1024), args.cpu_steps)
     print(json.dumps(out))

@@ -1,0 +1,118 @@
+// batch.hpp — many independent 2D two-material instances advanced together:
+// one launch per stage over a work table of (instance, tile) entries, walked
+// by persistent blocks.
+#pragma once
+
+#include "policy_fp32.hpp"
+#include "sweep.hpp"
+
+namespace dimhip {
+
+template<typename Pol>
+struct BatchInst {
+    Grid g;
+    typename Pol::Eos P;
+    BcArgs bc;
+    CflArgs cfl;
+    void* q[2];
+    StepCtl* ctl;
+};
+
+// work entry: x = instance, y/z = tile coordinates (sweeps) or ring /
+// cell range [y, z) (boundary and CFL passes)
+struct Work {
+    int inst;
+    int a;
+    int b;
+    int pad;
+};
+
+// schedule: 0 = cyclic (block k takes entries k, k+G, ...), 1 = contiguous
+// chunks (block k takes [k*chunk, (k+1)*chunk))
+__device__ __forceinline__ void work_range(int nwork, int schedule, int& beg, int& end, int& stride)
+{
+    if (schedule == 0) {
+        beg = blockIdx.x;
+        end = nwork;
+        stride = gridDim.x;
+    } else {
+        const int chunk = (nwork + gridDim.x - 1) / gridDim.x;
+        beg = blockIdx.x * chunk;
+        end = min(nwork, beg + chunk);
+        stride = 1;
+    }
+}
+
+template<typename Pol, int DIR, int L, int W, bool FUSE>
+__global__ void __launch_bounds__(L* W)
+batch_sweep_kernel(const BatchInst<Pol>* __restrict__ insts, const Work* __restrict__ work, int nwork, int schedule,
+                   int cur)
+{
+    int beg, end, stride;
+    work_range(nwork, schedule, beg, end, stride);
+    for (int w = beg; w < end; w += stride) {
+        const Work it = work[w];
+        const BatchInst<Pol>& I = insts[it.inst];
+        sweep_tile<Pol, 2, 2, DIR, L, W, 0, FUSE>(I.q[cur], I.q[cur ^ 1], I.g, I.P, I.ctl, I.cfl, it.a, it.b, 0);
+    }
+}
+
+template<typename Pol>
+__global__ void __launch_bounds__(256)
+batch_bc_kernel(const BatchInst<Pol>* __restrict__ insts, const Work* __restrict__ work, int nwork, int cur,
+                int end_step)
+{
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const Work it = work[w];
+        const BatchInst<Pol>& I = insts[it.inst];
+        const bool active = I.ctl->active;
+        for (long long t = it.a + (long long)threadIdx.x; active && t < it.b; t += blockDim.x)
+            bc_cell<Pol, 2, 2>(I.q[cur], I.g, I.P, I.bc, t);
+        // the entry that starts the instance's ring closes its step
+        if (end_step && it.a == 0 && threadIdx.x == 0 && active) end_step_ctl(I.ctl);
+    }
+}
+
+template<typename Pol>
+__global__ void __launch_bounds__(256)
+batch_cfl_kernel(const BatchInst<Pol>* __restrict__ insts, const Work* __restrict__ work, int nwork, int cur)
+{
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const Work it = work[w];
+        const BatchInst<Pol>& I = insts[it.inst];
+        double cand = bits_to_double(kInfBits);
+        for (long long c = it.a + (long long)threadIdx.x; c < it.b; c += blockDim.x) {
+            typename Pol::template Cell<2, 2> U;
+            Pol::template load<2, 2>(I.q[cur], I.g.pitch, c, U);
+            cand = smin(cand, Pol::template cfl_candidate<2, 2>(I.P, U, I.cfl));
+        }
+        block_min_atomic<256>(cand, &I.ctl->dtmin[I.ctl->slot]);
+    }
+}
+
+template<typename Pol>
+__global__ void batch_begin_step_kernel(const BatchInst<Pol>* __restrict__ insts, int n, int reset_current)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    StepCtl* ctl = insts[i].ctl;
+    if (reset_current) {
+        ctl->dtmin[ctl->slot] = kInfBits;
+        return;
+    }
+    const int s = ctl->slot;
+    const double m = bits_to_double(ctl->dtmin[s]);
+    int active = ctl->time < ctl->t_end - 1e-14;
+    const double dt_max = ctl->t_end - ctl->time;
+    const double dt = smin(dt_max, m);
+    if (active && !(dt > 0.0)) {
+        ctl->err |= ERR_NONPOSITIVE_DT;
+        active = 0;
+    }
+    ctl->dt = dt;
+    ctl->dt_max = dt_max;
+    ctl->active = active;
+    ctl->dtmin[s ^ 1] = kInfBits;
+}
+
+}  // namespace dimhip

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/dimhip.h"
+#include "batch.hpp"
 #include "kernels_fp64.hpp"
 #include "policy_fp32.hpp"
 
@@ -841,6 +842,473 @@ int dimhip_kat(const dimhip_desc* desc, int which, int dir, int count, const dou
     if (e != hipSuccess) return fail(nullptr, std::string("HIP error: ") + hipGetErrorString(e));
     if (rc) return fail(nullptr, "dimhip_kat: launch failed");
     if (herr) return fail(nullptr, "dim::hllc_flux_normal: invalid left density (non-finite)");
+    return 0;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched form: many 2D two-material FP32-mixed instances of uneven size,
+// one launch per stage over an (instance, tile) work table.
+// ===========================================================================
+namespace {
+
+using BPol = Fp32Mixed;
+using BInst = BatchInst<BPol>;
+
+// tile-shape variants of the batched sweeps (autotuned)
+constexpr int kBatchXL[2] = {128, 256};  // x sweep: positions per line; 256/L lines per block
+constexpr int kBatchYW[3] = {8, 16, 4};  // y sweep: lines (x) per block
+constexpr int kBatchYL[3] = {64, 32, 128};
+
+}  // namespace
+
+struct dimhip_batch {
+    std::vector<dimhip_ctx*> ctx;
+    hipStream_t stream = nullptr;
+    std::vector<BInst> h_insts;
+    BInst* d_insts = nullptr;
+    Work* d_work[4] = {nullptr, nullptr, nullptr, nullptr};  // x, y, bc, cfl
+    int nwork[4] = {0, 0, 0, 0};
+    int xv = 0, yv = 0, schedule = 1, blocks_per_cu = 4;
+    int n_cu = 256;
+    int cur = 0;
+    bool dt_ready = false;
+    double dt_ready_cfl = 0.0;
+    bool fusable = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    double graph_cfl = -1.0;
+    int graph_cur = -1;
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_pending;
+    size_t ev_next = 0;
+    double kt_sum[3] = {0, 0, 0};
+    long long kt_cnt[3] = {0, 0, 0};
+    std::string err;
+};
+
+namespace {
+
+int bfail(dimhip_batch* b, const std::string& msg)
+{
+    if (b) b->err = msg;
+    g_global_err = msg;
+    return 1;
+}
+
+#define BHIP_OK(b, expr)                                                                            \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess) return bfail((b), std::string("HIP error: ") + hipGetErrorString(e_) + \
+                                                     " at " #expr);                                 \
+    } while (0)
+
+int bev_begin(dimhip_batch* b)
+{
+    if (!b->timing) return -1;
+    if (b->ev_next + 2 > b->ev_pool.size())
+        for (int k = 0; k < 64; ++k) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return -1;
+            b->ev_pool.push_back(e);
+        }
+    const int i = (int)b->ev_next;
+    b->ev_next += 2;
+    hipEventRecord(b->ev_pool[i], b->stream);
+    return i;
+}
+
+void bev_end(dimhip_batch* b, int i, int cls)
+{
+    if (i < 0) return;
+    hipEventRecord(b->ev_pool[i + 1], b->stream);
+    b->ev_pending.push_back({cls, i});
+}
+
+void bev_collect(dimhip_batch* b)
+{
+    for (auto& pr : b->ev_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, b->ev_pool[pr.second], b->ev_pool[pr.second + 1]) == hipSuccess) {
+            b->kt_sum[pr.first] += ms;
+            b->kt_cnt[pr.first] += 1;
+        }
+    }
+    b->ev_pending.clear();
+    b->ev_next = 0;
+}
+
+void batch_drop_graph(dimhip_batch* b)
+{
+    if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+    b->graph_cfl = -1.0;
+    b->graph_cur = -1;
+}
+
+// work tables for the current variant choice
+int batch_build_work(dimhip_batch* b)
+{
+    std::vector<Work> w[4];
+    const int Lx = kBatchXL[b->xv], Wx = 256 / Lx, Tx = Lx - 6;
+    const int Wy = kBatchYW[b->yv], Ly = kBatchYL[b->yv], Ty = Ly - 6;
+    for (int i = 0; i < (int)b->ctx.size(); ++i) {
+        const Grid& g = b->ctx[i]->grid;
+        for (int by = 0; by < (g.N[1] + Wx - 1) / Wx; ++by)
+            for (int bx = 0; bx < (g.N[0] + Tx - 1) / Tx; ++bx) w[0].push_back({i, bx, by, 0});
+        for (int by = 0; by < (g.N[1] + Ty - 1) / Ty; ++by)
+            for (int bx = 0; bx < (g.N[0] + Wy - 1) / Wy; ++bx) w[1].push_back({i, bx, by, 0});
+        const long long ring = b->ctx[i]->bc.face_start[4];
+        for (long long a = 0; a < ring; a += 1024) w[2].push_back({i, (int)a, (int)std::min<long long>(ring, a + 1024), 0});
+        for (long long a = 0; a < g.ncells; a += 16384)
+            w[3].push_back({i, (int)a, (int)std::min<long long>(g.ncells, a + 16384), 0});
+    }
+    for (int k = 0; k < 4; ++k) {
+        hipFree(b->d_work[k]);
+        b->d_work[k] = nullptr;
+        b->nwork[k] = (int)w[k].size();
+        BHIP_OK(b, hipMalloc(&b->d_work[k], sizeof(Work) * std::max<size_t>(1, w[k].size())));
+        BHIP_OK(b, hipMemcpy(b->d_work[k], w[k].data(), sizeof(Work) * w[k].size(), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int batch_sync_insts(dimhip_batch* b, double cfl)
+{
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+        dimhip_ctx* c = b->ctx[i];
+        BInst& I = b->h_insts[i];
+        I.g = c->grid;
+        I.P = c->eosm;
+        I.bc = c->bc;
+        I.cfl = make_cfl_args(c, cfl, false);
+        I.q[0] = c->q[0];
+        I.q[1] = c->q[1];
+        I.ctl = c->ctl;
+    }
+    BHIP_OK(b, hipMemcpyAsync(b->d_insts, b->h_insts.data(), sizeof(BInst) * b->h_insts.size(),
+                              hipMemcpyHostToDevice, b->stream));
+    return 0;
+}
+
+int sweep_blocks(const dimhip_batch* b, int nwork)
+{
+    return std::max(1, std::min(nwork, b->n_cu * b->blocks_per_cu));
+}
+
+template<int DIR, int L, int W, bool FUSE>
+void launch_bsweep(dimhip_batch* b, int k)
+{
+    const dim3 blk = DIR == 0 ? dim3(L, W, 1) : dim3(W, L, 1);
+    batch_sweep_kernel<BPol, DIR, L, W, FUSE><<<sweep_blocks(b, b->nwork[k]), blk, 0, b->stream>>>(
+        b->d_insts, b->d_work[k], b->nwork[k], b->schedule, b->cur);
+}
+
+void batch_sweep_x(dimhip_batch* b)
+{
+    if (b->xv == 0) launch_bsweep<0, 128, 2, false>(b, 0);
+    else launch_bsweep<0, 256, 1, false>(b, 0);
+}
+
+void batch_sweep_y(dimhip_batch* b, bool fuse)
+{
+    switch (b->yv * 2 + (fuse ? 1 : 0)) {
+        case 0: launch_bsweep<1, 64, 8, false>(b, 1); break;
+        case 1: launch_bsweep<1, 64, 8, true>(b, 1); break;
+        case 2: launch_bsweep<1, 32, 16, false>(b, 1); break;
+        case 3: launch_bsweep<1, 32, 16, true>(b, 1); break;
+        case 4: launch_bsweep<1, 128, 4, false>(b, 1); break;
+        default: launch_bsweep<1, 128, 4, true>(b, 1); break;
+    }
+}
+
+void batch_enqueue_step(dimhip_batch* b, double cfl)
+{
+    const int n = (int)b->ctx.size();
+    const int tb = (n + 63) / 64;
+    if (!b->dt_ready || b->dt_ready_cfl != cfl) {
+        const int e = bev_begin(b);
+        batch_begin_step_kernel<BPol><<<tb, 64, 0, b->stream>>>(b->d_insts, n, 1);
+        batch_cfl_kernel<BPol><<<std::max(1, std::min(b->nwork[3], 4096)), 256, 0, b->stream>>>(
+            b->d_insts, b->d_work[3], b->nwork[3], b->cur);
+        bev_end(b, e, KC_CFL);
+    }
+    batch_begin_step_kernel<BPol><<<tb, 64, 0, b->stream>>>(b->d_insts, n, 0);
+    const int bcb = std::max(1, std::min(b->nwork[2], 2048));
+    int e = bev_begin(b);
+    batch_sweep_x(b);
+    bev_end(b, e, KC_SWEEP);
+    b->cur ^= 1;
+    e = bev_begin(b);
+    batch_bc_kernel<BPol><<<bcb, 256, 0, b->stream>>>(b->d_insts, b->d_work[2], b->nwork[2], b->cur, 0);
+    bev_end(b, e, KC_BC);
+    e = bev_begin(b);
+    batch_sweep_y(b, b->fusable);
+    bev_end(b, e, KC_SWEEP);
+    b->cur ^= 1;
+    e = bev_begin(b);
+    batch_bc_kernel<BPol><<<bcb, 256, 0, b->stream>>>(b->d_insts, b->d_work[2], b->nwork[2], b->cur, 1);
+    bev_end(b, e, KC_BC);
+    b->dt_ready = b->fusable;
+    b->dt_ready_cfl = cfl;
+    for (auto* c : b->ctx) c->cur = b->cur;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dimhip_batch_last_error(const dimhip_batch* b) { return b ? b->err.c_str() : g_global_err.c_str(); }
+
+int dimhip_batch_create(const dimhip_desc* descs, int count, dimhip_batch** out)
+{
+    if (!descs || !out || count <= 0) return bfail(nullptr, "dimhip_batch_create: bad argument");
+    auto b = std::make_unique<dimhip_batch>();
+    for (int i = 0; i < count; ++i) {
+        const dimhip_desc& d = descs[i];
+        if (d.dim != 2 || d.nmat != 2 || d.precision != DIMHIP_FP32_MIXED || d.time_update != DIMHIP_SPLIT)
+            return bfail(nullptr, "dimhip_batch_create: the batched form runs 2D, two-material, FP32-mixed, split steps");
+        if (d.device != descs[0].device) return bfail(nullptr, "dimhip_batch_create: instances must share a device");
+        for (int a = 0; a < 2; ++a)
+            if (d.bc_lo[a] == DIMHIP_BC_NONREFLECTIVE || d.bc_hi[a] == DIMHIP_BC_NONREFLECTIVE) b->fusable = false;
+        dimhip_ctx* c = nullptr;
+        if (dimhip_create(&d, &c)) {
+            for (auto* x : b->ctx) dimhip_destroy(x);
+            return bfail(nullptr, std::string("instance ") + std::to_string(i) + ": " + g_global_err);
+        }
+        b->ctx.push_back(c);
+    }
+    BHIP_OK(nullptr, hipSetDevice(descs[0].device));
+    hipDeviceProp_t prop;
+    BHIP_OK(nullptr, hipGetDeviceProperties(&prop, descs[0].device));
+    b->n_cu = prop.multiProcessorCount;
+    BHIP_OK(nullptr, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    b->h_insts.resize(count);
+    BHIP_OK(nullptr, hipMalloc(&b->d_insts, sizeof(BInst) * count));
+    if (batch_build_work(b.get()) || batch_sync_insts(b.get(), 0.8)) return 1;
+    BHIP_OK(nullptr, hipStreamSynchronize(b->stream));
+    *out = b.release();
+    return 0;
+}
+
+int dimhip_batch_destroy(dimhip_batch* b)
+{
+    if (!b) return 0;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    batch_drop_graph(b);
+    for (auto e : b->ev_pool) hipEventDestroy(e);
+    for (int k = 0; k < 4; ++k) hipFree(b->d_work[k]);
+    hipFree(b->d_insts);
+    for (auto* c : b->ctx) dimhip_destroy(c);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return 0;
+}
+
+int dimhip_batch_upload(dimhip_batch* b, int i, const double* packed, size_t ncells)
+{
+    if (!b || i < 0 || i >= (int)b->ctx.size()) return bfail(b, "dimhip_batch_upload: bad instance");
+    BHIP_OK(b, hipStreamSynchronize(b->stream));
+    if (dimhip_upload(b->ctx[i], packed, ncells)) return bfail(b, b->ctx[i]->err);
+    if (b->ctx[i]->cur != b->cur) {
+        // keep every instance on the same buffer parity
+        copy_state(b->ctx[i], b->ctx[i]->q[b->ctx[i]->cur], b->ctx[i]->q[b->cur]);
+        b->ctx[i]->cur = b->cur;
+        BHIP_OK(b, hipStreamSynchronize(b->ctx[i]->stream));
+    }
+    b->dt_ready = false;
+    return 0;
+}
+
+int dimhip_batch_download(dimhip_batch* b, int i, double* packed, size_t ncells)
+{
+    if (!b || i < 0 || i >= (int)b->ctx.size()) return bfail(b, "dimhip_batch_download: bad instance");
+    BHIP_OK(b, hipStreamSynchronize(b->stream));
+    if (dimhip_download(b->ctx[i], packed, ncells)) return bfail(b, b->ctx[i]->err);
+    return 0;
+}
+
+int dimhip_batch_configure(dimhip_batch* b, int x_variant, int y_variant, int schedule, int blocks_per_cu)
+{
+    if (!b) return bfail(b, "dimhip_batch_configure: null batch");
+    if (x_variant < 0 || x_variant > 1 || y_variant < 0 || y_variant > 2 || schedule < 0 || schedule > 1 ||
+        blocks_per_cu < 1 || blocks_per_cu > 64)
+        return bfail(b, "dimhip_batch_configure: variant out of range");
+    BHIP_OK(b, hipStreamSynchronize(b->stream));
+    b->xv = x_variant;
+    b->yv = y_variant;
+    b->schedule = schedule;
+    b->blocks_per_cu = blocks_per_cu;
+    batch_drop_graph(b);
+    return batch_build_work(b);
+}
+
+int dimhip_batch_advance(dimhip_batch* b, int nsteps, double cfl, double t_end, int use_graph, double* times,
+                         long long* steps)
+{
+    if (!b || !times) return bfail(b, "dimhip_batch_advance: null argument");
+    if (cfl <= 0.0) return bfail(b, "dim::compute_dt_cfl: cfl must be positive");
+    BHIP_OK(b, hipSetDevice(b->ctx[0]->desc.device));
+    const int n = (int)b->ctx.size();
+    std::vector<long long> step0(n);
+    for (int i = 0; i < n; ++i) {
+        set_mode_kernel<<<1, 1, 0, b->stream>>>(b->ctx[i]->ctl, 1, t_end, times[i], 1);
+    }
+    if (batch_sync_insts(b, cfl)) return 1;
+    for (int i = 0; i < n; ++i)
+        BHIP_OK(b, hipMemcpyAsync(b->ctx[i]->ctl_host, b->ctx[i]->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, b->stream));
+    BHIP_OK(b, hipStreamSynchronize(b->stream));
+    for (int i = 0; i < n; ++i) step0[i] = b->ctx[i]->ctl_host->step;
+    if (nsteps > 0 && (!b->dt_ready || b->dt_ready_cfl != cfl)) {
+        batch_enqueue_step(b, cfl);
+        --nsteps;
+    }
+    if (use_graph && !b->timing && b->dt_ready && nsteps >= 2) {
+        if (!b->graph_exec || b->graph_cfl != cfl || b->graph_cur != b->cur) {
+            batch_drop_graph(b);
+            const int cur0 = b->cur;
+            BHIP_OK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            batch_enqueue_step(b, cfl);
+            batch_enqueue_step(b, cfl);
+            BHIP_OK(b, hipStreamEndCapture(b->stream, &b->graph));
+            BHIP_OK(b, hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+            b->graph_cfl = cfl;
+            b->graph_cur = b->cur;
+            if (b->cur != cur0) return bfail(b, "dimhip_batch_advance: graph does not return to its buffer");
+        }
+        while (nsteps >= 2) {
+            BHIP_OK(b, hipGraphLaunch(b->graph_exec, b->stream));
+            nsteps -= 2;
+        }
+    }
+    while (nsteps > 0) {
+        batch_enqueue_step(b, cfl);
+        --nsteps;
+    }
+    BHIP_OK(b, hipGetLastError());
+    for (int i = 0; i < n; ++i)
+        BHIP_OK(b, hipMemcpyAsync(b->ctx[i]->ctl_host, b->ctx[i]->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, b->stream));
+    BHIP_OK(b, hipStreamSynchronize(b->stream));
+    bev_collect(b);
+    for (int i = 0; i < n; ++i) {
+        if (check_device_error(b->ctx[i])) return bfail(b, std::string("instance ") + std::to_string(i) + ": " + b->ctx[i]->err);
+        times[i] = b->ctx[i]->ctl_host->time;
+        if (steps) steps[i] = b->ctx[i]->ctl_host->step - step0[i];
+    }
+    return 0;
+}
+
+int dimhip_batch_set_timing(dimhip_batch* b, int enable)
+{
+    if (!b) return bfail(b, "dimhip_batch_set_timing: null batch");
+    BHIP_OK(b, hipStreamSynchronize(b->stream));
+    bev_collect(b);
+    b->timing = enable != 0;
+    for (int k = 0; k < 3; ++k) {
+        b->kt_sum[k] = 0.0;
+        b->kt_cnt[k] = 0;
+    }
+    return 0;
+}
+
+int dimhip_batch_kernel_times(dimhip_batch* b, double* mean_ms, long long* counts)
+{
+    if (!b || !mean_ms) return bfail(b, "dimhip_batch_kernel_times: null argument");
+    BHIP_OK(b, hipStreamSynchronize(b->stream));
+    bev_collect(b);
+    for (int k = 0; k < 3; ++k) {
+        mean_ms[k] = b->kt_cnt[k] ? b->kt_sum[k] / (double)b->kt_cnt[k] : 0.0;
+        if (counts) counts[k] = b->kt_cnt[k];
+    }
+    return 0;
+}
+
+// Times every (x tile, y tile, schedule, blocks per CU) choice over
+// trial_steps steps from the current state, restores the state, keeps the
+// fastest choice and describes the trials as JSON in report.
+int dimhip_batch_autotune(dimhip_batch* b, double cfl, int trial_steps, char* report, size_t report_len)
+{
+    if (!b || trial_steps < 1) return bfail(b, "dimhip_batch_autotune: bad argument");
+    const int n = (int)b->ctx.size();
+    BHIP_OK(b, hipStreamSynchronize(b->stream));
+    // back up every instance's state and control block
+    std::vector<void*> backup(n, nullptr);
+    std::vector<StepCtl> ctl_backup(n);
+    for (int i = 0; i < n; ++i) {
+        dimhip_ctx* c = b->ctx[i];
+        const size_t bytes = (size_t)c->grid.pitch * c->cell_bytes;
+        BHIP_OK(b, hipMalloc(&backup[i], bytes));
+        BHIP_OK(b, hipMemcpy(backup[i], c->q[b->cur], bytes, hipMemcpyDeviceToDevice));
+        BHIP_OK(b, hipMemcpy(&ctl_backup[i], c->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost));
+    }
+    const int cur0 = b->cur;
+    const bool ready0 = b->dt_ready;
+    std::vector<double> times(n, 0.0);
+    hipEvent_t e0, e1;
+    BHIP_OK(b, hipEventCreate(&e0));
+    BHIP_OK(b, hipEventCreate(&e1));
+    std::string js = "{\"trials\": [";
+    double best = 1e300;
+    int bx = 0, by = 0, bs = 1, bb = 4;
+    const int bpcs[3] = {2, 4, 8};
+    bool first = true;
+    for (int xv = 0; xv < 2; ++xv)
+        for (int yv = 0; yv < 3; ++yv)
+            for (int sc = 0; sc < 2; ++sc)
+                for (int bi = 0; bi < 3; ++bi) {
+                    if (dimhip_batch_configure(b, xv, yv, sc, bpcs[bi])) return 1;
+                    for (int i = 0; i < n; ++i) times[i] = ctl_backup[i].time;
+                    // one untimed step settles dt, then the timed trial
+                    if (dimhip_batch_advance(b, 1, cfl, 1e300, 0, times.data(), nullptr)) return 1;
+                    hipEventRecord(e0, b->stream);
+                    for (int s = 0; s < trial_steps; ++s) batch_enqueue_step(b, cfl);
+                    hipEventRecord(e1, b->stream);
+                    BHIP_OK(b, hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    hipEventElapsedTime(&ms, e0, e1);
+                    const double per = ms / trial_steps;
+                    char buf[160];
+                    snprintf(buf, sizeof buf, "%s{\"x\": %d, \"y\": %d, \"schedule\": %d, \"blocks_per_cu\": %d, \"ms\": %.5f}",
+                             first ? "" : ", ", xv, yv, sc, bpcs[bi], per);
+                    js += buf;
+                    first = false;
+                    if (per < best) {
+                        best = per;
+                        bx = xv;
+                        by = yv;
+                        bs = sc;
+                        bb = bpcs[bi];
+                    }
+                    // restore
+                    BHIP_OK(b, hipStreamSynchronize(b->stream));
+                    b->cur = cur0;
+                    for (int i = 0; i < n; ++i) {
+                        dimhip_ctx* c = b->ctx[i];
+                        BHIP_OK(b, hipMemcpy(c->q[cur0], backup[i], (size_t)c->grid.pitch * c->cell_bytes,
+                                             hipMemcpyDeviceToDevice));
+                        BHIP_OK(b, hipMemcpy(c->ctl, &ctl_backup[i], sizeof(StepCtl), hipMemcpyHostToDevice));
+                        c->cur = cur0;
+                    }
+                    b->dt_ready = ready0;
+                }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    for (auto* p : backup) hipFree(p);
+    if (dimhip_batch_configure(b, bx, by, bs, bb)) return 1;
+    char tail[256];
+    snprintf(tail, sizeof tail,
+             "], \"best\": {\"x\": %d, \"y\": %d, \"schedule\": %d, \"blocks_per_cu\": %d, \"ms\": %.5f}, "
+             "\"x_tiles\": [[128, 2], [256, 1]], \"y_tiles\": [[8, 64], [16, 32], [4, 128]]}",
+             bx, by, bs, bb, best);
+    js += tail;
+    if (report && report_len) {
+        strncpy(report, js.c_str(), report_len - 1);
+        report[report_len - 1] = 0;
+    }
     return 0;
 }
 

@@ -68,6 +68,7 @@ template<int NT>
 __device__ __forceinline__ void block_min_atomic(double v, unsigned long long* target)
 {
     __shared__ double red[NT / 64 > 0 ? NT / 64 : 1];
+    __syncthreads();  // the block may reuse red[] for its next tile
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         const double o = __shfl_xor(v, off, 64);
@@ -125,9 +126,9 @@ __device__ __forceinline__ void xcd_block(int& bx, int& by, int& bz)
 // pass, so the interior minimum is the next step's dt).
 // ---------------------------------------------------------------------------
 template<typename Pol, int D, int M, int DIR, int L, int W, int MODE, bool FUSE_CFL>
-__global__ void __launch_bounds__(L* W)
-sweep_kernel(const void* __restrict__ qin, void* __restrict__ qout, Grid g, typename Pol::Eos P,
-             StepCtl* __restrict__ ctl, CflArgs cflargs)
+__device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* __restrict__ qout, const Grid& g,
+                                           const typename Pol::Eos& P, StepCtl* __restrict__ ctl,
+                                           const CflArgs& cflargs, int bx, int by, int bz)
 {
     using R = typename Pol::R;
     using St = typename Pol::template St<D, M>;
@@ -140,6 +141,7 @@ sweep_kernel(const void* __restrict__ qin, void* __restrict__ qout, Grid g, type
     constexpr int T = L - 6;
     __shared__ R sh[NREC * S];
 
+    __syncthreads();  // a block may run several tiles (batched launches): LDS reuse
     const double dt = ctl->dt;
     const double dxs = g.dx[DIR];
 
@@ -156,8 +158,6 @@ sweep_kernel(const void* __restrict__ qin, void* __restrict__ qout, Grid g, type
     const int slot_r = (DIR == 0) ? slot + 1 : slot + W;
 
     const int n = g.N[DIR];
-    int bx, by, bz;
-    xcd_block(bx, by, bz);
     int idx[3] = {0, 0, 0};
     bool line_ok;
     int i;
@@ -348,6 +348,17 @@ sweep_kernel(const void* __restrict__ qin, void* __restrict__ qout, Grid g, type
     if (FUSE_CFL) block_min_atomic<S>(cand, &ctl->dtmin[ctl->slot ^ 1]);
 }
 
+// one tile per block, blocks in XCD-aware order
+template<typename Pol, int D, int M, int DIR, int L, int W, int MODE, bool FUSE_CFL>
+__global__ void __launch_bounds__(L* W)
+sweep_kernel(const void* __restrict__ qin, void* __restrict__ qout, Grid g, typename Pol::Eos P,
+             StepCtl* __restrict__ ctl, CflArgs cflargs)
+{
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    sweep_tile<Pol, D, M, DIR, L, W, MODE, FUSE_CFL>(qin, qout, g, P, ctl, cflargs, bx, by, bz);
+}
+
 // ---------------------------------------------------------------------------
 // Ring boundary pass (boundary.hpp:165-226).  Each ring cell is owned by the
 // face of its lowest boundary axis; with extent >= 3 on every axis longer
@@ -422,15 +433,13 @@ __device__ __forceinline__ void nonreflective_state(const typename Pol::Eos& P, 
     out = st;
 }
 
+// boundary treatment of ring cell number t (0 <= t < bc.face_start[2D])
 template<typename Pol, int D, int M>
-__global__ void __launch_bounds__(256)
-bc_kernel(void* __restrict__ q, Grid g, typename Pol::Eos P, BcArgs bc, StepCtl* __restrict__ ctl, int end_step)
+__device__ __forceinline__ void bc_cell(void* __restrict__ q, const Grid& g, const typename Pol::Eos& P,
+                                        const BcArgs& bc, long long t)
 {
     using Cell = typename Pol::template Cell<D, M>;
-    const long long total = bc.face_start[2 * D];
-    const bool active = ctl->active;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; active && t < total;
-         t += (long long)gridDim.x * blockDim.x) {
+    {
         int face = 0;
         while (face < 2 * D - 1 && t >= bc.face_start[face + 1]) ++face;
         long long local = t - bc.face_start[face];
@@ -485,11 +494,25 @@ bc_kernel(void* __restrict__ q, Grid g, typename Pol::Eos P, BcArgs bc, StepCtl*
         }
         Pol::template store<D, M>(q, g.pitch, lin, st);
     }
-    if (end_step && blockIdx.x == 0 && threadIdx.x == 0 && active) {
-        ctl->time += ctl->dt;
-        ctl->step += 1;
-        ctl->slot ^= 1;
-    }
+}
+
+__device__ __forceinline__ void end_step_ctl(StepCtl* ctl)
+{
+    ctl->time += ctl->dt;
+    ctl->step += 1;
+    ctl->slot ^= 1;
+}
+
+template<typename Pol, int D, int M>
+__global__ void __launch_bounds__(256)
+bc_kernel(void* __restrict__ q, Grid g, typename Pol::Eos P, BcArgs bc, StepCtl* __restrict__ ctl, int end_step)
+{
+    const long long total = bc.face_start[2 * D];
+    const bool active = ctl->active;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; active && t < total;
+         t += (long long)gridDim.x * blockDim.x)
+        bc_cell<Pol, D, M>(q, g, P, bc, t);
+    if (end_step && blockIdx.x == 0 && threadIdx.x == 0 && active) end_step_ctl(ctl);
 }
 
 // compute_dt_cfl over every cell into ctl->dtmin[ctl->slot]

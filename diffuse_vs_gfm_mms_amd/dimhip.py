@@ -82,6 +82,16 @@ def load_library():
         "dimhip_kernel_times": (C.c_int, [vp, dp, llp]),
         "dimhip_kat": (C.c_int, [C.POINTER(Desc), C.c_int, C.c_int, C.c_int, dp, dp, dp, dp]),
         "dimhip_last_error": (C.c_char_p, [vp]),
+        "dimhip_batch_create": (C.c_int, [C.POINTER(Desc), C.c_int, C.POINTER(vp)]),
+        "dimhip_batch_destroy": (C.c_int, [vp]),
+        "dimhip_batch_upload": (C.c_int, [vp, C.c_int, dp, C.c_size_t]),
+        "dimhip_batch_download": (C.c_int, [vp, C.c_int, dp, C.c_size_t]),
+        "dimhip_batch_configure": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "dimhip_batch_autotune": (C.c_int, [vp, C.c_double, C.c_int, C.c_char_p, C.c_size_t]),
+        "dimhip_batch_advance": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_int, dp, llp]),
+        "dimhip_batch_set_timing": (C.c_int, [vp, C.c_int]),
+        "dimhip_batch_kernel_times": (C.c_int, [vp, dp, llp]),
+        "dimhip_batch_last_error": (C.c_char_p, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -95,7 +105,10 @@ def exported_symbols():
     return ["dimhip_version", "dimhip_create", "dimhip_destroy", "dimhip_upload", "dimhip_download",
             "dimhip_step", "dimhip_advance", "dimhip_compute_dt", "dimhip_sweep", "dimhip_apply_bc",
             "dimhip_conservation", "dimhip_synchronize", "dimhip_set_timing", "dimhip_kernel_times",
-            "dimhip_kat", "dimhip_last_error"]
+            "dimhip_kat", "dimhip_last_error", "dimhip_batch_create", "dimhip_batch_destroy",
+            "dimhip_batch_upload", "dimhip_batch_download", "dimhip_batch_configure", "dimhip_batch_autotune",
+            "dimhip_batch_advance", "dimhip_batch_set_timing", "dimhip_batch_kernel_times",
+            "dimhip_batch_last_error"]
 
 
 def _dp(a):
@@ -241,3 +254,77 @@ def kat(which, dim, materials, count, in0, in1=None, d=0, lambda_model="kapila",
     if rc != 0:
         raise RuntimeError(lib.dimhip_last_error(None).decode())
     return out0[:size0], out1[:count]
+
+
+class Batch:
+    """Many 2D two-material FP32-mixed instances advanced together
+    (include/dimhip.h, batched form)."""
+
+    def __init__(self, cases, device=0):
+        self.lib = load_library()
+        self.cases = list(cases)
+        descs = (Desc * len(self.cases))()
+        for i, c in enumerate(self.cases):
+            descs[i] = make_desc(c.dim, c.N, c.dx, c.materials, c.bc_lo, c.bc_hi, c.lambda_model,
+                                 c.alpha_source_floor, "split", "fp32_mixed", device)
+        h = C.c_void_p()
+        if self.lib.dimhip_batch_create(descs, len(self.cases), C.byref(h)) != 0:
+            raise RuntimeError(self.lib.dimhip_batch_last_error(None).decode())
+        self.h = h
+        self.times = np.zeros(len(self.cases))
+        for i, c in enumerate(self.cases):
+            self.upload(i, c.U)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.lib.dimhip_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dimhip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, i, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        self._check(self.lib.dimhip_batch_upload(self.h, i, _dp(U), self.cases[i].ncells))
+
+    def download(self, i):
+        out = np.empty((self.cases[i].ncells, self.cases[i].nv))
+        self._check(self.lib.dimhip_batch_download(self.h, i, _dp(out), self.cases[i].ncells))
+        return out
+
+    def configure(self, x_variant, y_variant, schedule, blocks_per_cu):
+        self._check(self.lib.dimhip_batch_configure(self.h, x_variant, y_variant, schedule, blocks_per_cu))
+
+    def autotune(self, cfl, trial_steps=3):
+        import json
+        buf = C.create_string_buffer(16384)
+        self._check(self.lib.dimhip_batch_autotune(self.h, cfl, trial_steps, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def advance(self, nsteps, cfl, t_end=1e300, use_graph=True):
+        steps = (C.c_longlong * len(self.cases))()
+        self._check(self.lib.dimhip_batch_advance(self.h, nsteps, cfl, t_end, int(use_graph),
+                                                  _dp(self.times), steps))
+        return self.times.copy(), list(steps)
+
+    def set_timing(self, on):
+        self._check(self.lib.dimhip_batch_set_timing(self.h, int(on)))
+
+    def kernel_times(self):
+        ms = np.zeros(3)
+        cnt = (C.c_longlong * 3)()
+        self._check(self.lib.dimhip_batch_kernel_times(self.h, _dp(ms), cnt))
+        return ms, list(cnt)

@@ -195,3 +195,47 @@ def bubble_collapse_3d(n=512):
                         shock_axis=0, shock_position=0.2, thickness=2.0 * h,
                         name=f"bubble_collapse_3d_{n}")
     return case
+
+
+def batch_instances(count=64, seed=20261203, nx_range=(256, 1024), ny_range=(256, 512)):
+    """Seeded shock-interface instances of uneven size (instance i uses
+    numpy's PCG64 generator seeded with seed + i).
+
+    Per instance: Nx ~ U{nx_range}, Ny ~ U{ny_range}, height H = 1 and
+    length Nx/Ny (square cells); ambient gas a and interface gas b ideal with
+    gamma ~ U[1.3, 1.7]; density ratio rho_b/rho_a ~ U[0.1, 10]; shock Mach
+    ~ U[1.1, 3.0] from Rankine-Hugoniot in gas a, entering from the right;
+    interface radius ~ U[0.1, 0.3] H; half the instances a circular bubble,
+    half a sinusoidally perturbed planar interface x = x0 + A sin(2 pi k y/H);
+    transmissive in x, reflective in y; interface thickness 2 cells.
+    """
+    cases = []
+    for i in range(count):
+        rng = np.random.default_rng(seed + i)
+        nx = int(rng.integers(nx_range[0], nx_range[1] + 1))
+        ny = int(rng.integers(ny_range[0], ny_range[1] + 1))
+        H = 1.0
+        Lx = nx / ny * H
+        ga, gb = rng.uniform(1.3, 1.7, 2)
+        ratio = rng.uniform(0.1, 10.0)
+        mach = rng.uniform(1.1, 3.0)
+        r = rng.uniform(0.1, 0.3) * H
+        rho1, p1 = 1.0, 1.0e5
+        rho2, p2, u2 = rankine_hugoniot(ga, mach, rho1, p1)
+        mats = [("ideal_gas", float(ga), 0.0), ("ideal_gas", float(gb), 0.0)]
+        x0 = 0.4 * Lx
+        shock_x = min(x0 + r + 0.1 * Lx, 0.9 * Lx)
+        if i % 2 == 0:
+            shape = None
+        else:
+            amp = 0.5 * r
+            k = int(rng.integers(1, 4))
+            shape = (lambda X, x0=x0, amp=amp, k=k: X[0] - (x0 + amp * np.sin(2.0 * np.pi * k * X[1] / H)))
+        c = shock_bubble((nx, ny), (0.0, 0.0), (Lx, H), mats, 0, 1,
+                         pre=(rho1, (0.0, 0.0), p1), post=(rho2, (-u2, 0.0), p2),
+                         bubble=(rho1 * ratio, (0.0, 0.0), p1), centre=(x0, 0.5 * H), radius=r,
+                         shock_axis=0, shock_position=shock_x, thickness=2.0 * H / ny,
+                         bc_lo=("transmissive", "reflective"), bc_hi=("transmissive", "reflective"),
+                         name=f"batch_{i}_{nx}x{ny}", bubble_shape=shape)
+        cases.append(c)
+    return cases

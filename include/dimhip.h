@@ -155,6 +155,40 @@ int dimhip_kat(const dimhip_desc* desc, int which, int dir, int count, const dou
 
 const char* dimhip_last_error(const dimhip_ctx* ctx);
 
+/* ---- batched form ---------------------------------------------------------
+ * Many independent 2D two-material FP32-mixed instances of uneven size
+ * advanced together: each stage (CFL, x sweep, ring BCs, y sweep, ring BCs)
+ * is one launch over a work table of (instance, tile) entries walked by
+ * persistent blocks; every instance keeps its own dt and time on the device
+ * (the reference runs such sweeps as separate processes, one config each,
+ * tools/quant/runner.hpp:246-371).  Results equal running each instance
+ * alone through dimhip_advance. */
+typedef struct dimhip_batch dimhip_batch;
+
+int dimhip_batch_create(const dimhip_desc* descs, int count, dimhip_batch** out);
+int dimhip_batch_destroy(dimhip_batch* batch);
+int dimhip_batch_upload(dimhip_batch* batch, int instance, const double* packed, size_t ncells);
+int dimhip_batch_download(dimhip_batch* batch, int instance, double* packed, size_t ncells);
+
+/* Tile shape and schedule: x_variant 0 = 128 x 2, 1 = 256 x 1 (positions x
+ * lines); y_variant 0 = 8 x 64, 1 = 16 x 32, 2 = 4 x 128 (lines x positions);
+ * schedule 0 = cyclic, 1 = contiguous chunks of the work table per block;
+ * blocks_per_cu = persistent blocks launched per compute unit. */
+int dimhip_batch_configure(dimhip_batch* batch, int x_variant, int y_variant, int schedule, int blocks_per_cu);
+
+/* Times every configuration over trial_steps steps from the current state,
+ * restores the state, keeps the fastest and writes a JSON report. */
+int dimhip_batch_autotune(dimhip_batch* batch, double cfl, int trial_steps, char* report, size_t report_len);
+
+/* Up to nsteps steps of every instance (dt_max = t_end - time per instance);
+ * times[i] in/out, steps[i] out (may be NULL). */
+int dimhip_batch_advance(dimhip_batch* batch, int nsteps, double cfl, double t_end, int use_graph,
+                         double* times, long long* steps);
+
+int dimhip_batch_set_timing(dimhip_batch* batch, int enable);
+int dimhip_batch_kernel_times(dimhip_batch* batch, double* mean_ms, long long* counts);
+const char* dimhip_batch_last_error(const dimhip_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif
