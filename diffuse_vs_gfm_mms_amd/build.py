@@ -1,4 +1,6 @@
-"""Builds the gfx950 HIP library in-tree: diffuse_vs_gfm_mms_amd/_lib/libdimhip.so.
+"""Builds in-tree: the gfx950 HIP library diffuse_vs_gfm_mms_amd/_lib/libdimhip.so
+and the C++17 case drivers _lib/dim_main_{1,2,3}d that run the reference's DIM
+case files on it.
 
     python -m diffuse_vs_gfm_mms_amd.build
 """
@@ -52,5 +54,33 @@ def build_library(verbose=False):
     return OUT
 
 
+HOST_DIR = os.path.join(HERE, "host")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
+def driver_path(dim):
+    return os.path.join(OUT_DIR, f"dim_main_{dim}d")
+
+
+def build_drivers():
+    """g++ -std=c++17 -O2 (the reference's own host flags), linked to libdimhip.so."""
+    if not os.path.exists(OUT):
+        raise RuntimeError("build the HIP library first")
+    outs = []
+    for dim in (1, 2, 3):
+        out = driver_path(dim)
+        cmd = ["g++", "-std=c++17", "-O2", f"-DDIM_={dim}", "-I", INCLUDE, "-I", HOST_DIR,
+               os.path.join(HOST_DIR, "dim_main.cpp"), "-o", out + ".tmp", "-L", OUT_DIR, "-ldimhip",
+               "-Wl,-rpath,$ORIGIN", "-pthread"]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"g++ failed building {out}")
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+    return outs
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(*build_drivers())

@@ -14,6 +14,7 @@
 
 #include "../../include/dimhip.h"
 #include "batch.hpp"
+#include "io_kernels.hpp"
 #include "kernels_fp64.hpp"
 #include "policy_fp32.hpp"
 
@@ -50,6 +51,16 @@ struct dimhip_ctx {
     double* partial = nullptr;
     double* totals = nullptr;
     int partial_blocks = 0;
+    // driver I/O: primitive rows in, CSV row values out (grown on demand)
+    double* io_buf = nullptr;
+    size_t io_doubles = 0;
+    // boundary flux: per-block partials, last result, running integral
+    long long flux_lines[4] = {0, 0, 0, 0};
+    int flux_blocks = 0;
+    double* flux_partial = nullptr;
+    double* flux_out = nullptr;
+    double* flux_integral = nullptr;
+    bool track_flux = false;
     bool dt_ready = false;       // ctl->dtmin[slot] holds the CFL minimum of the current state
     double dt_ready_cfl = 0.0;   // the cfl it was computed with
     std::string err;
@@ -132,6 +143,9 @@ struct Ops {
     virtual void pack(dimhip_ctx* c, bool in, double* staging, void* q, long long first, long long count) const = 0;
     virtual int kat(const EosParams& P, int which, int dir, int count, const double* in0, const double* in1,
                     double* out0, double* out1, int* err, hipStream_t s) const = 0;
+    virtual void prim_to_packed(dimhip_ctx* c, const double* prim, double* packed, long long count) const = 0;
+    virtual void output_rows(dimhip_ctx* c, const double* packed, double* rows, long long count) const = 0;
+    virtual void boundary_flux(dimhip_ctx* c, const void* q, double* integral) const = 0;
 };
 
 CflArgs make_cfl_args(const dimhip_ctx* c, double cfl, bool unsplit)
@@ -240,6 +254,29 @@ struct OpsImpl final : Ops {
             if (in) pack_in_kernel<Fp32Mixed, D, M><<<blocks, 256, 0, c->stream>>>(c->eosm, staging, q, first, count, c->grid.pitch);
             else pack_out_kernel<Fp32Mixed, D, M><<<blocks, 256, 0, c->stream>>>(c->eosm, q, staging, first, count, c->grid.pitch);
         }
+    }
+
+    void prim_to_packed(dimhip_ctx* c, const double* prim, double* packed, long long count) const override
+    {
+        prim_to_packed_kernel<D, M><<<grid_blocks(count, 256), 256, 0, c->stream>>>(c->eos, prim, packed, count);
+    }
+
+    void output_rows(dimhip_ctx* c, const double* packed, double* rows, long long count) const override
+    {
+        output_rows_kernel<D, M><<<grid_blocks(count, 256), 256, 0, c->stream>>>(c->eos, packed, rows, count);
+    }
+
+    void boundary_flux(dimhip_ctx* c, const void* q, double* integral) const override
+    {
+        const long long* L = c->flux_lines;
+        if (c->desc.precision == DIMHIP_FP64)
+            boundary_flux_kernel<Fp64, D, M><<<c->flux_blocks, 256, 0, c->stream>>>(q, c->grid, c->eos, c->eos, L[0],
+                                                                                      L[1], L[2], L[3], c->flux_partial);
+        else
+            boundary_flux_kernel<Fp32Mixed, D, M><<<c->flux_blocks, 256, 0, c->stream>>>(
+                q, c->grid, c->eosm, c->eos, L[0], L[1], L[2], L[3], c->flux_partial);
+        finish_boundary_flux_kernel<<<1, 64, 0, c->stream>>>(c->flux_partial, c->flux_blocks, FluxTotals<D, M>::NO,
+                                                              c->flux_out, integral, c->ctl);
     }
 
     int kat(const EosParams& P, int which, int dir, int count, const double* in0, const double* in1, double* out0,
@@ -406,6 +443,7 @@ void enqueue_step(dimhip_ctx* c, double cfl, double dt_max_host)
         c->cur ^= 1;
     }
     c->dt_ready = fuse;
+    if (c->track_flux) c->ops->boundary_flux(c, state_ptr(c, c->cur), c->flux_integral);
 }
 
 int check_device_error(dimhip_ctx* c)
@@ -431,6 +469,28 @@ void drop_graph(dimhip_ctx* c)
     c->graph = nullptr;
     c->graph_cfl = -1.0;
     c->graph_cur = -1;
+}
+
+int ensure_staging(dimhip_ctx* c, long long cells)
+{
+    if (c->staging_cells >= cells) return 0;
+    hipFree(c->staging);
+    c->staging = nullptr;
+    c->staging_cells = 0;
+    HIP_OK(c, hipMalloc(&c->staging, sizeof(double) * (size_t)cells * c->nv));
+    c->staging_cells = cells;
+    return 0;
+}
+
+int ensure_io(dimhip_ctx* c, size_t doubles)
+{
+    if (c->io_doubles >= doubles) return 0;
+    hipFree(c->io_buf);
+    c->io_buf = nullptr;
+    c->io_doubles = 0;
+    HIP_OK(c, hipMalloc(&c->io_buf, sizeof(double) * doubles));
+    c->io_doubles = doubles;
+    return 0;
 }
 
 }  // namespace
@@ -524,6 +584,22 @@ int dimhip_create(const dimhip_desc* desc, dimhip_ctx** out)
     c->partial_blocks = (int)std::min<long long>(1024, std::max<long long>(1, g.ncells / 4096));
     HIP_OK(nullptr, hipMalloc(&c->partial, sizeof(double) * (size_t)c->partial_blocks * 16));
     HIP_OK(nullptr, hipMalloc(&c->totals, sizeof(double) * 16));
+    {
+        long long acc_lines = 0;
+        for (int d = 0; d < desc->dim; ++d) {
+            c->flux_lines[d] = acc_lines;
+            long long cnt = 1;
+            for (int e = 0; e < desc->dim; ++e)
+                if (e != d) cnt *= g.N[e];
+            acc_lines += cnt;
+        }
+        for (int d = desc->dim; d < 4; ++d) c->flux_lines[d] = acc_lines;
+        c->flux_blocks = (int)std::min<long long>(64, std::max<long long>(1, (acc_lines + 255) / 256));
+    }
+    HIP_OK(nullptr, hipMalloc(&c->flux_partial, sizeof(double) * (size_t)c->flux_blocks * 16));
+    HIP_OK(nullptr, hipMalloc(&c->flux_out, sizeof(double) * 16));
+    HIP_OK(nullptr, hipMalloc(&c->flux_integral, sizeof(double) * 16));
+    HIP_OK(nullptr, hipMemsetAsync(c->flux_integral, 0, sizeof(double) * 16, c->stream));
     init_ctl_kernel<<<1, 1, 0, c->stream>>>(c->ctl);
     HIP_OK(nullptr, hipGetLastError());
     HIP_OK(nullptr, hipStreamSynchronize(c->stream));
@@ -545,6 +621,10 @@ int dimhip_destroy(dimhip_ctx* c)
     hipFree(c->staging);
     hipFree(c->partial);
     hipFree(c->totals);
+    hipFree(c->io_buf);
+    hipFree(c->flux_partial);
+    hipFree(c->flux_out);
+    hipFree(c->flux_integral);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -556,12 +636,7 @@ int dimhip_upload(dimhip_ctx* c, const double* packed, size_t ncells)
     if ((long long)ncells != c->grid.ncells) return fail(c, "dim::advance_one_step: grid size mismatch");
     HIP_OK(c, hipSetDevice(c->desc.device));
     const long long chunk = std::min<long long>(c->grid.ncells, 1ll << 22);
-    if (c->staging_cells < chunk) {
-        hipFree(c->staging);
-        c->staging = nullptr;
-        HIP_OK(c, hipMalloc(&c->staging, sizeof(double) * (size_t)chunk * c->nv));
-        c->staging_cells = chunk;
-    }
+    if (ensure_staging(c, chunk)) return 1;
     for (long long first = 0; first < c->grid.ncells; first += chunk) {
         const long long cnt = std::min(chunk, c->grid.ncells - first);
         HIP_OK(c, hipMemcpyAsync(c->staging, packed + first * c->nv, sizeof(double) * (size_t)(cnt * c->nv),
@@ -580,12 +655,7 @@ int dimhip_download(dimhip_ctx* c, double* packed, size_t ncells)
     if ((long long)ncells != c->grid.ncells) return fail(c, "dim::advance_one_step: grid size mismatch");
     HIP_OK(c, hipSetDevice(c->desc.device));
     const long long chunk = std::min<long long>(c->grid.ncells, 1ll << 22);
-    if (c->staging_cells < chunk) {
-        hipFree(c->staging);
-        c->staging = nullptr;
-        HIP_OK(c, hipMalloc(&c->staging, sizeof(double) * (size_t)chunk * c->nv));
-        c->staging_cells = chunk;
-    }
+    if (ensure_staging(c, chunk)) return 1;
     for (long long first = 0; first < c->grid.ncells; first += chunk) {
         const long long cnt = std::min(chunk, c->grid.ncells - first);
         c->ops->pack(c, false, c->staging, c->q[c->cur], first, cnt);
@@ -728,6 +798,81 @@ int dimhip_conservation(dimhip_ctx* c, double* out)
     HIP_OK(c, hipGetLastError());
     const int no = c->desc.nmat + 1 + c->desc.dim + 1;
     HIP_OK(c, hipMemcpyAsync(out, c->totals, sizeof(double) * no, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dimhip_upload_primitive(dimhip_ctx* c, const double* prim, size_t ncells)
+{
+    if (!c || !prim) return fail(c, "dimhip_upload_primitive: null argument");
+    if ((long long)ncells != c->grid.ncells) return fail(c, "dim::advance_one_step: grid size mismatch");
+    HIP_OK(c, hipSetDevice(c->desc.device));
+    const int np = c->nv + 1;
+    const long long chunk = std::min<long long>(c->grid.ncells, 1ll << 22);
+    if (ensure_staging(c, chunk) || ensure_io(c, (size_t)chunk * np)) return 1;
+    for (long long first = 0; first < c->grid.ncells; first += chunk) {
+        const long long cnt = std::min(chunk, c->grid.ncells - first);
+        HIP_OK(c, hipMemcpyAsync(c->io_buf, prim + first * np, sizeof(double) * (size_t)(cnt * np),
+                                 hipMemcpyHostToDevice, c->stream));
+        c->ops->prim_to_packed(c, c->io_buf, c->staging, cnt);
+        c->ops->pack(c, true, c->staging, c->q[c->cur], first, cnt);
+        HIP_OK(c, hipGetLastError());
+        HIP_OK(c, hipStreamSynchronize(c->stream));
+    }
+    c->dt_ready = false;
+    return 0;
+}
+
+int dimhip_output_rows(dimhip_ctx* c, double* rows, size_t ncells)
+{
+    if (!c || !rows) return fail(c, "dimhip_output_rows: null argument");
+    if ((long long)ncells != c->grid.ncells) return fail(c, "dim::write_csv: grid size mismatch");
+    HIP_OK(c, hipSetDevice(c->desc.device));
+    const int nr = 3 + c->desc.dim + 3 * c->desc.nmat;
+    const long long chunk = std::min<long long>(c->grid.ncells, 1ll << 22);
+    if (ensure_staging(c, chunk) || ensure_io(c, (size_t)chunk * nr)) return 1;
+    for (long long first = 0; first < c->grid.ncells; first += chunk) {
+        const long long cnt = std::min(chunk, c->grid.ncells - first);
+        c->ops->pack(c, false, c->staging, c->q[c->cur], first, cnt);
+        c->ops->output_rows(c, c->staging, c->io_buf, cnt);
+        HIP_OK(c, hipGetLastError());
+        HIP_OK(c, hipMemcpyAsync(rows + first * nr, c->io_buf, sizeof(double) * (size_t)(cnt * nr),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(c, hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+int dimhip_boundary_flux(dimhip_ctx* c, double* out)
+{
+    if (!c || !out) return fail(c, "dimhip_boundary_flux: null argument");
+    HIP_OK(c, hipSetDevice(c->desc.device));
+    c->ops->boundary_flux(c, state_ptr(c, c->cur), nullptr);
+    HIP_OK(c, hipGetLastError());
+    const int no = c->desc.nmat + 1 + c->desc.dim + 1;
+    HIP_OK(c, hipMemcpyAsync(out, c->flux_out, sizeof(double) * no, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dimhip_track_boundary_flux(dimhip_ctx* c, int enable)
+{
+    if (!c) return fail(c, "dimhip_track_boundary_flux: null context");
+    HIP_OK(c, hipSetDevice(c->desc.device));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    drop_graph(c);
+    HIP_OK(c, hipMemsetAsync(c->flux_integral, 0, sizeof(double) * 16, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    c->track_flux = enable != 0;
+    return 0;
+}
+
+int dimhip_boundary_flux_integral(dimhip_ctx* c, double* out)
+{
+    if (!c || !out) return fail(c, "dimhip_boundary_flux_integral: null argument");
+    HIP_OK(c, hipSetDevice(c->desc.device));
+    const int no = c->desc.nmat + 1 + c->desc.dim + 1;
+    HIP_OK(c, hipMemcpyAsync(out, c->flux_integral, sizeof(double) * no, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     return 0;
 }

@@ -77,6 +77,11 @@ def load_library():
         "dimhip_sweep": (C.c_int, [vp, C.c_int, C.c_double]),
         "dimhip_apply_bc": (C.c_int, [vp]),
         "dimhip_conservation": (C.c_int, [vp, dp]),
+        "dimhip_boundary_flux": (C.c_int, [vp, dp]),
+        "dimhip_track_boundary_flux": (C.c_int, [vp, C.c_int]),
+        "dimhip_boundary_flux_integral": (C.c_int, [vp, dp]),
+        "dimhip_upload_primitive": (C.c_int, [vp, dp, C.c_size_t]),
+        "dimhip_output_rows": (C.c_int, [vp, dp, C.c_size_t]),
         "dimhip_synchronize": (C.c_int, [vp]),
         "dimhip_set_timing": (C.c_int, [vp, C.c_int]),
         "dimhip_kernel_times": (C.c_int, [vp, dp, llp]),
@@ -104,7 +109,8 @@ def load_library():
 def exported_symbols():
     return ["dimhip_version", "dimhip_create", "dimhip_destroy", "dimhip_upload", "dimhip_download",
             "dimhip_step", "dimhip_advance", "dimhip_compute_dt", "dimhip_sweep", "dimhip_apply_bc",
-            "dimhip_conservation", "dimhip_synchronize", "dimhip_set_timing", "dimhip_kernel_times",
+            "dimhip_conservation", "dimhip_boundary_flux", "dimhip_track_boundary_flux",
+            "dimhip_boundary_flux_integral", "dimhip_upload_primitive", "dimhip_output_rows", "dimhip_synchronize", "dimhip_set_timing", "dimhip_kernel_times",
             "dimhip_kat", "dimhip_last_error", "dimhip_batch_create", "dimhip_batch_destroy",
             "dimhip_batch_upload", "dimhip_batch_download", "dimhip_batch_configure", "dimhip_batch_autotune",
             "dimhip_batch_advance", "dimhip_batch_set_timing", "dimhip_batch_kernel_times",
@@ -221,6 +227,33 @@ class Solver:
     def conservation(self):
         out = np.zeros(self.nmat + 1 + self.dim + 1)
         self._check(self.lib.dimhip_conservation(self.h, _dp(out)))
+        return out
+
+    def boundary_flux(self):
+        """[flux mass_k, mass, mom_d, E] through the domain boundary of the current state."""
+        out = np.zeros(self.nmat + 1 + self.dim + 1)
+        self._check(self.lib.dimhip_boundary_flux(self.h, _dp(out)))
+        return out
+
+    def track_boundary_flux(self, enable):
+        self._check(self.lib.dimhip_track_boundary_flux(self.h, int(bool(enable))))
+
+    def boundary_flux_integral(self):
+        out = np.zeros(self.nmat + 1 + self.dim + 1)
+        self._check(self.lib.dimhip_boundary_flux_integral(self.h, _dp(out)))
+        return out
+
+    def upload_primitive(self, prim):
+        """prim: (ncells, 2*nmat + dim + 1) rows rho[nmat], alpha[nmat], vel[dim], p."""
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        if prim.size != self.ncells * (2 * self.nmat + self.dim + 1):
+            raise RuntimeError("dim::advance_one_step: grid size mismatch")
+        self._check(self.lib.dimhip_upload_primitive(self.h, _dp(prim), self.ncells))
+
+    def output_rows(self):
+        """(ncells, 3 + dim + 3*nmat): rho, vel, p, e, alpha, rho_mat, mass per cell."""
+        out = np.empty((self.ncells, 3 + self.dim + 3 * self.nmat))
+        self._check(self.lib.dimhip_output_rows(self.h, _dp(out), self.ncells))
         return out
 
     def synchronize(self):

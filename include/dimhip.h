@@ -126,6 +126,36 @@ int dimhip_apply_bc(dimhip_ctx* ctx);
  * tree sums: out = [mass_k * V (nmat), mass * V, mom_d * V (dim), E * V]. */
 int dimhip_conservation(dimhip_ctx* ctx, double* out);
 
+/* Physical flux through the domain boundary of the current state
+ * (compute_dim_boundary_flux, conservation_report.hpp:253-305): for every
+ * line normal to a face, +area * F(last cell) - area * F(first cell), summed
+ * in FP64 in a fixed order on the device.  out = [flux mass_k (nmat), mass,
+ * mom_d (dim), E]. */
+int dimhip_boundary_flux(dimhip_ctx* ctx, double* out);
+
+/* With enable != 0, every step taken by dimhip_step / dimhip_advance ends
+ * with the report's accumulate_fluxes (run_dim.hpp:113-125,
+ * conservation_report.hpp:506-520): integral += dt * boundary_flux(new
+ * state), kept on the device.  Enabling or disabling resets the integral to
+ * zero.  dimhip_boundary_flux_integral reads it (layout of
+ * dimhip_boundary_flux). */
+int dimhip_track_boundary_flux(dimhip_ctx* ctx, int enable);
+int dimhip_boundary_flux_integral(dimhip_ctx* ctx, double* out);
+
+/* Initial state from primitives: prim holds ncells rows of
+ * rho[nmat], alpha[nmat], vel[dim], p (2*nmat + dim + 1 doubles); the device
+ * stores prim_to_cons of each row (primitives.hpp:166-212), exactly as the
+ * reference's initialisers do per cell (dim_initial_conditions.hpp:531-628). */
+int dimhip_upload_primitive(dimhip_ctx* ctx, const double* prim, size_t ncells);
+
+/* The values of the solution CSV rows (dim_write_csv.hpp:78-121), computed on
+ * the device in FP64 from the current state: per cell
+ *   rho, vel[dim], p, e, alpha[nmat], rho_mat[nmat], mass[nmat]
+ * (3 + dim + 3*nmat doubles) with P = cons_to_prim(U), rho = total_density(U),
+ * e = max(safe_div(E - 0.5*rho*|vel|^2, rho), 0).  The cell coordinates are
+ * left to the caller. */
+int dimhip_output_rows(dimhip_ctx* ctx, double* rows, size_t ncells);
+
 /* Wait for the context's stream. */
 int dimhip_synchronize(dimhip_ctx* ctx);
 

@@ -217,5 +217,24 @@ class Checker:
         return out
 
 
+def ref_initial_state(path, dim, n_index=0):
+    """The reference's initial state of a case file (load_config +
+    initialise_dim_from_config), packed (ncells, 2*nmat + dim)."""
+    lib = C.CDLL(REF_LIB)
+    fn = lib.ref_initial_state
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
+                   C.POINTER(C.c_int)]
+    lib.ref_last_error.restype = C.c_char_p
+    n = C.c_longlong(0)
+    m = C.c_int(0)
+    if fn(path.encode(), dim, n_index, None, C.byref(n), C.byref(m)) != 0:
+        raise RuntimeError(lib.ref_last_error().decode())
+    U = np.zeros((n.value, 2 * m.value + dim))
+    if fn(path.encode(), dim, n_index, _dp(U), C.byref(n), C.byref(m)) != 0:
+        raise RuntimeError(lib.ref_last_error().decode())
+    return U
+
+
 def ref_available():
     return os.path.exists(REF_LIB)

@@ -18,7 +18,10 @@
 #include <string>
 #include <vector>
 
+#include "src/app/material/material_builder.hpp"
 #include "src/dim/solver/advance_step.hpp"
+#include "src/io/config_loader.hpp"
+#include "src/setup/dim_initial_conditions.hpp"
 
 namespace {
 
@@ -333,9 +336,50 @@ int advance_t(const RefParams* p, const int* N, const double* dx, const int* bc_
     });
 }
 
+// The reference's initial state of a case file: load_config,
+// build_dim_material_params and initialise_dim_from_config for resolution
+// N_list[n_index], packed.  U == nullptr only reports the cell count and nmat.
+template<int DIM>
+int initial_state_t(const char* path, int n_index, double* U, long long* ncells, int* nmat)
+{
+    try {
+        const Config<DIM> cfg = load_config<DIM>(path);
+        if (n_index < 0 || n_index >= (int)cfg.N_list.size()) {
+            g_err = "ref_initial_state: resolution index out of range";
+            return 1;
+        }
+        const auto N = cfg.N_list[n_index];
+        const dim::EOSParams params = build_dim_material_params(cfg.materials);
+        long long total = 1;
+        for (int d = 0; d < DIM; ++d) total *= N[d];
+        *ncells = total;
+        *nmat = params.nmat();
+        if (!U) return 0;
+        std::vector<dim::State<DIM>> grid;
+        dim::initialise_dim_from_config<DIM>(grid, cfg, N, params);
+        RefParams rp{};
+        rp.dim = DIM;
+        rp.nmat = params.nmat();
+        for (long long i = 0; i < total; ++i) pack<DIM>(&rp, grid[i], U + i * nv_of(&rp));
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+int ref_initial_state(const char* path, int dim, int n_index, double* U, long long* ncells, int* nmat)
+{
+    if (dim == 1) return initial_state_t<1>(path, n_index, U, ncells, nmat);
+    if (dim == 2) return initial_state_t<2>(path, n_index, U, ncells, nmat);
+    if (dim == 3) return initial_state_t<3>(path, n_index, U, ncells, nmat);
+    g_err = "ref_initial_state: dim must be 1, 2 or 3";
+    return 1;
+}
 
 const char* ref_last_error() { return g_err.c_str(); }
 

@@ -1,0 +1,214 @@
+"""The case driver (dim_main_<D>d) against the reference's own driver
+(oracle/_ref/mm_main_<D>d, built from the reference's sources) on the same
+case files: every solution CSV must be byte-identical, the runtime reports
+must agree on steps and final time, and the conservation reports
+(SOLVER_CONSERVATION=1) must agree to FP64 summation-order noise.
+
+Also checks the I/O entry points on their own: dimhip_output_rows against the
+oracle's cons_to_prim, dimhip_upload_primitive against the oracle's
+prim_to_cons, and dimhip_boundary_flux against the flux through the faces
+recomputed from oracle primitives (conservation_report.hpp:253-305).
+"""
+import csv
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle.pyoracle import make_params, ref_available
+from diffuse_vs_gfm_mms_amd import dimhip, workloads as W
+from diffuse_vs_gfm_mms_amd.build import driver_path
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CASES = os.path.join(ROOT, "tests", "cases")
+REF_MAIN = os.path.join(ROOT, "oracle", "_ref", "mm_main_{}d")
+
+CASE_FILES = {
+    "tube_1d_diffuse.txt": 1,
+    "water_air_1d_sharp.txt": 1,
+    "helium_bubble_2d.txt": 2,
+    "oblique_2d.txt": 2,
+    "coated_bubble_2d.txt": 2,
+    "box_regions_2d.txt": 2,
+    "bubble_collapse_3d.txt": 3,
+}
+
+
+def write_case(tmp_path, name, outdir):
+    text = open(os.path.join(CASES, name)).read().replace("OUTDIR", str(outdir))
+    p = tmp_path / f"{outdir.name}_{name}"
+    p.write_text(text)
+    return str(p)
+
+
+def run(exe, case, env_extra=None, cwd=None):
+    env = dict(os.environ)
+    env.update(env_extra or {})
+    print("running", exe, case, flush=True)
+    res = subprocess.run([exe, case], capture_output=True, text=True, env=env, timeout=300, cwd=cwd)
+    assert res.returncode == 0, res.stdout + res.stderr
+    return res.stdout
+
+
+def files_under(d):
+    out = {}
+    for base, _, names in os.walk(d):
+        for n in names:
+            out[os.path.relpath(os.path.join(base, n), d)] = os.path.join(base, n)
+    return out
+
+
+def report(path):
+    kv = {}
+    for line in open(path):
+        if " = " in line:
+            k, v = line.rstrip("\n").split(" = ", 1)
+            kv[k] = v
+    return kv
+
+
+needs_ref = pytest.mark.skipif(not all(os.path.exists(REF_MAIN.format(d)) for d in (1, 2, 3)),
+                               reason="reference drivers (oracle/_ref) not built")
+
+
+@needs_ref
+@pytest.mark.parametrize("name", sorted(CASE_FILES))
+def test_driver_csv_matches_reference_driver(tmp_path, name):
+    dim = CASE_FILES[name]
+    ours, theirs = tmp_path / "ours", tmp_path / "ref"
+    out_a = run(driver_path(dim), write_case(tmp_path, name, ours))
+    out_b = run(REF_MAIN.format(dim), write_case(tmp_path, name, theirs))
+    fa, fb = files_under(ours), files_under(theirs)
+    assert sorted(fa) == sorted(fb)
+    csvs = [k for k in fa if k.endswith(".csv")]
+    assert csvs
+    for k in csvs:
+        a, b = open(fa[k], "rb").read(), open(fb[k], "rb").read()
+        assert a == b, f"{name}: {k} differs"
+    runtime = [k for k in fa if k.endswith("_runtime.txt")]
+    assert runtime
+    for k in runtime:
+        ra, rb = report(fa[k]), report(fb[k])
+        for key in ("interface_method", "dimension", "N", "cells", "steps", "final_time"):
+            assert ra[key] == rb[key], (name, key, ra[key], rb[key])
+    # console lines other than timings are the reference's
+    keep = lambda s: [l.replace(str(ours), "D").replace(str(theirs), "D") for l in s.splitlines()]
+    assert keep(out_a) == keep(out_b)
+
+
+@needs_ref
+@pytest.mark.parametrize("name,interval", [("helium_bubble_2d.txt", "1"), ("coated_bubble_2d.txt", "3"),
+                                           ("tube_1d_diffuse.txt", "5")])
+def test_conservation_report_matches_reference_driver(tmp_path, name, interval):
+    dim = CASE_FILES[name]
+    env = {"SOLVER_CONSERVATION": "1", "SOLVER_CONSERVATION_INTERVAL": interval}
+    ours, theirs = tmp_path / "ours", tmp_path / "ref"
+    run(driver_path(dim), write_case(tmp_path, name, ours), env)
+    run(REF_MAIN.format(dim), write_case(tmp_path, name, theirs), env)
+    fa, fb = files_under(ours), files_under(theirs)
+    keys = [k for k in fa if k.endswith("_conservation.csv")]
+    assert keys and sorted(keys) == sorted(k for k in fb if k.endswith("_conservation.csv"))
+    for key in keys:
+        compare_conservation(name, dim, fa[key], fb[key])
+
+
+def compare_conservation(name, dim, path_a, path_b):
+    ra = list(csv.reader(open(path_a)))
+    rb = list(csv.reader(open(path_b)))
+    assert ra[0] == rb[0] and len(ra) == len(rb)
+    A = np.array(ra[1:], dtype=np.float64)
+    B = np.array(rb[1:], dtype=np.float64)
+    assert np.array_equal(A[:, :2], B[:, :2])  # step, time
+    # totals and integrals are FP64 sums in another order: compare each column
+    # against the size of the totals of its quantity (all momentum components
+    # share one scale: a component whose total cancels to zero still sums
+    # terms of the other components' size)
+    header = ra[0]
+    col = {h: j for j, h in enumerate(header)}
+    mom = max(np.max(np.abs(B[:, col[f"momentum{d}"]])) for d in range(dim))
+    base = {"mass": np.max(np.abs(B[:, col["mass"]])), "energy": np.max(np.abs(B[:, col["energy"]]))}
+    for j, h in enumerate(header[2:], start=2):
+        if h.startswith("eps_") or h.startswith("normalized_") or h.startswith("near_zero"):
+            continue
+        q = h.split("_")[-1] if "_" in h else h
+        S = mom if q.startswith("momentum") else base["energy"] if q == "energy" else base["mass"]
+        # the files hold 6 significant digits: one unit of the last digit is
+        # up to 1e-5 of the value
+        tol = 1e-12 * S + 1e-5 * np.abs(B[:, j])
+        assert np.all(np.abs(A[:, j] - B[:, j]) <= tol), (name, h)
+
+
+def test_output_rows_and_primitive_upload_match_oracle(oracle):
+    c = W.helium_shock_bubble_2d(96, 40)
+    p = make_params(2, c.materials)
+    with dimhip.Solver.from_case(c, precision="fp64") as s:
+        s.advance(5, c.cfl, 1.0)
+        U = s.download()
+        rows = s.output_rows()
+        P = oracle.cons_to_prim(p, U)
+        nm = c.nmat
+        rho = U[:, :nm].sum(1)
+        v2 = P[:, 2 * nm] * P[:, 2 * nm] + P[:, 2 * nm + 1] * P[:, 2 * nm + 1]
+        e = np.maximum((U[:, nm + 2] - 0.5 * rho * v2) / rho, 0.0)
+        expect = np.column_stack([rho, P[:, 2 * nm:2 * nm + 2], P[:, -1], e, P[:, nm:2 * nm], P[:, :nm], U[:, :nm]])
+        assert np.array_equal(rows.view(np.uint64), expect.view(np.uint64))
+        # primitive upload = oracle prim_to_cons
+        s.upload_primitive(P)
+        U2 = s.download()
+        assert np.array_equal(U2.view(np.uint64), oracle.prim_to_cons(p, P).view(np.uint64))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
+def test_boundary_flux_and_its_integral(oracle, precision):
+    c = W.helium_shock_bubble_2d(64, 32)
+    p = make_params(2, c.materials)
+    nx, ny = c.N
+    dx, dy = c.dx
+    with dimhip.Solver.from_case(c, precision=precision) as s:
+        s.advance(3, c.cfl, 1.0)
+        U = s.download()
+        got = s.boundary_flux()
+        P = oracle.cons_to_prim(p, U)
+        g = U.reshape(ny, nx, -1)
+        q = P.reshape(ny, nx, -1)
+        expect = np.zeros(6)
+
+        def add(cells, prims, d, area):
+            un = prims[:, 4 + d]
+            rho = cells[:, :2].sum(1)
+            expect[0:2] += (area * cells[:, :2] * un[:, None]).sum(0)
+            expect[2] += (area * cells[:, :2] * un[:, None]).sum()
+            for e in range(2):
+                expect[3 + e] += (area * (rho * prims[:, 4 + e] * un + (prims[:, -1] if e == d else 0.0))).sum()
+            expect[5] += (area * (cells[:, 4] + prims[:, -1]) * un).sum()
+
+        add(g[:, -1], q[:, -1], 0, dy)
+        add(g[:, 0], q[:, 0], 0, -dy)
+        add(g[-1, :], q[-1, :], 1, dx)
+        add(g[0, :], q[0, :], 1, -dx)
+        scale = np.abs(expect).max()
+        assert np.allclose(got, expect, rtol=1e-12 if precision == "fp64" else 1e-6, atol=1e-12 * scale)
+        # the integral accumulates dt * flux of each new state
+        s.track_boundary_flux(True)
+        assert np.all(s.boundary_flux_integral() == 0.0)
+        acc = np.zeros(6)
+        t = 0.0
+        for _ in range(3):
+            dt = s.step(c.cfl, 1.0)
+            acc += dt * s.boundary_flux()
+        assert np.allclose(s.boundary_flux_integral(), acc, rtol=1e-13, atol=1e-14 * np.abs(acc).max())
+        # device loop and graph replays accumulate the same way
+        s.track_boundary_flux(True)
+        before = s.download()
+        t, n = s.advance(4, c.cfl, 1.0, 0.0, use_graph=True)
+        integ = s.boundary_flux_integral()
+        s.upload(before)
+        s.track_boundary_flux(True)
+        acc = np.zeros(6)
+        for _ in range(4):
+            dt = s.step(c.cfl, 1.0)
+            acc += dt * s.boundary_flux()
+        assert np.allclose(integ, acc, rtol=1e-13, atol=1e-14 * np.abs(acc).max())
