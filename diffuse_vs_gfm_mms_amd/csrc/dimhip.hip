@@ -154,6 +154,7 @@ CflArgs make_cfl_args(const dimhip_ctx* c, double cfl, bool unsplit)
     for (int d = 0; d < 3; ++d) {
         a.dx[d] = c->grid.dx[d];
         a.cfl_dx[d] = cfl * c->grid.dx[d];
+        a.inv_cfl_dx[d] = 1.0 / a.cfl_dx[d];
     }
     a.cfl = cfl;
     a.unsplit = unsplit ? 1 : 0;

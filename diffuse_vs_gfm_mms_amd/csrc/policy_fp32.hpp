@@ -13,7 +13,11 @@
 //    cancellation bites, in FP32 elsewhere;
 //  * the CFL candidate cfl*dx/(|u|+c) and its reduction are FP64.
 // Divisions use the hardware reciprocal (v_rcp_f32, 1 ulp) and square roots
-// v_sqrt_f32 / v_rsq_f32.
+// v_sqrt_f32 / v_rsq_f32.  Floors are v_max_f32 / v_min_f32, for which a NaN
+// operand yields the other operand (the floor); states are finite by
+// construction (every stored or half-step state has been through repair), so
+// the reference's finite_or guards are kept only where a state is repaired
+// and in the Riemann solver's input check.
 #pragma once
 
 #include "policy_fp64.hpp"
@@ -37,22 +41,22 @@ struct EosMixed {
 
 namespace f32 {
 
-__device__ __forceinline__ float fmaxs(float a, float b) { return (a < b) ? b : a; }
-__device__ __forceinline__ float fmins(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float fmaxs(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ float fmins(float a, float b) { return __builtin_fminf(a, b); }
 template<typename T>
 __device__ __forceinline__ bool ffinite(T x) { return __builtin_isfinite(x); }
 __device__ __forceinline__ float ffinite_or(float x, float fb = 0.f) { return ffinite(x) ? x : fb; }
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+// |d| raised to tol, keeping the sign of d (numerical_safety.hpp:38-63)
 __device__ __forceinline__ float safe_denom_f(float d, float tol = 1e-10f)
 {
-    if (!ffinite(d) || __builtin_fabsf(d) < tol) return (d < 0.f) ? -tol : tol;
-    return d;
+    return __builtin_copysignf(__builtin_fmaxf(__builtin_fabsf(d), tol), d);
 }
 __device__ __forceinline__ float safe_div_f(float n, float d, float tol = 1e-10f)
 {
-    return ffinite_or(n) * rcp(safe_denom_f(d, tol));
+    return n * rcp(safe_denom_f(d, tol));
 }
 
 template<int D, int M>
@@ -89,10 +93,10 @@ struct Cell {
 template<int M>
 __device__ __forceinline__ void sanitise(float* alpha, float floor)
 {
-    float sum = 0.f;
+    float sum = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-        alpha[k] = fmaxs(ffinite_or(alpha[k], floor), floor);
+        alpha[k] = fmaxs(alpha[k], floor);
         sum += alpha[k];
     }
     if (!ffinite(sum) || sum <= 0.f) {
@@ -123,7 +127,7 @@ __device__ __forceinline__ void decode_alpha(const float* a, float* alpha)
             alpha[0] = 1.f + a[0];
         }
     } else {
-        float sum = 0.f;
+        float sum = -0.f;
 #pragma unroll
         for (int k = 0; k < M - 1; ++k) {
             alpha[k] = a[k];
@@ -153,8 +157,8 @@ __device__ __forceinline__ void encode_alpha(const float* alpha, float* a)
 template<int M>
 __device__ __forceinline__ float bshift(const EosMixed& P, const float* from, const float* to)
 {
-    if (!P.stiff) return 0.f;
-    float s = 0.f;
+    if (!P.stiff) return -0.f;
+    float s = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) s += P.b[k] * (from[k] - to[k]);
     return s;
@@ -163,11 +167,35 @@ __device__ __forceinline__ float bshift(const EosMixed& P, const float* from, co
 template<int M>
 __device__ __forceinline__ float bdot(const EosMixed& P, const float* alpha)
 {
-    if (!P.stiff) return 0.f;
-    float s = 0.f;
+    if (!P.stiff) return -0.f;
+    float s = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) s += P.b[k] * alpha[k];
     return s;
+}
+
+// The 1e-12 floor of cons_to_prim / prim_to_cons on fractions that are
+// already non-negative and sum to one: raising a fraction to 1e-12 moves the
+// sum by at most 1e-12 * M, far below half an FP32 ulp of 1, so the
+// renormalisation is the identity in FP32 and only the floor remains.  For
+// three or more materials the general sanitise runs.
+template<int M>
+__device__ __forceinline__ void floor_fractions(float* alpha)
+{
+    if constexpr (M <= 2) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) alpha[k] = fmaxs(alpha[k], 1e-12f);
+    } else {
+        sanitise<M>(alpha, 1e-12f);
+    }
+}
+
+// a stored two-material code is valid when |a| <= 1/2 (NaN is not)
+template<int M>
+__device__ __forceinline__ bool valid_code(const float* a)
+{
+    if constexpr (M == 2) return __builtin_fabsf(a[0]) <= 0.5f;
+    else return false;
 }
 
 template<int M>
@@ -183,7 +211,7 @@ __device__ __forceinline__ bool interface_cell(const float* alpha)
 template<int M>
 __device__ __forceinline__ float pressure_shifted(const EosMixed& P, float rho_e_hat, const float* alpha)
 {
-    float S = 0.f;
+    float S = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) S += fmaxs(alpha[k], 0.f) * P.inv_gm1[k];
     return rho_e_hat * rcp(S);
@@ -192,7 +220,7 @@ __device__ __forceinline__ float pressure_shifted(const EosMixed& P, float rho_e
 template<int M>
 __device__ __forceinline__ float pressure_shifted_d(const EosMixed& P, double rho_e_hat, const float* alpha)
 {
-    double S = 0.0;
+    double S = -0.0;
 #pragma unroll
     for (int k = 0; k < M; ++k) S += (double)fmaxs(alpha[k], 0.f) * P.inv_gm1_d[k];
     return (float)(rho_e_hat / S);
@@ -203,13 +231,14 @@ template<int M>
 __device__ __forceinline__ float energy_shifted(const EosMixed& P, float p, const float* alpha)
 {
     const float pk = fmaxs(p, 1e-10f);
-    float s = 0.f;
+    float s = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) s += fmaxs(alpha[k], 0.f) * (pk * P.inv_gm1[k]);
     return s;
 }
 
-// cons_to_prim (primitives.hpp:111-162) with the closed-form pressure
+// cons_to_prim (primitives.hpp:111-162) with the closed-form pressure, on a
+// state with a valid fraction code (stored, half-step and face states)
 template<int D, int M>
 __device__ __forceinline__ void cons_to_prim(const EosMixed& P, const St<D, M>& U, Pr<D, M>& R)
 {
@@ -217,100 +246,111 @@ __device__ __forceinline__ void cons_to_prim(const EosMixed& P, const St<D, M>& 
     decode_alpha<M>(U.a, ad);
 #pragma unroll
     for (int k = 0; k < M; ++k) R.alpha[k] = ad[k];
-    sanitise<M>(R.alpha, 1e-12f);
-    float rho = 0.f;
+    floor_fractions<M>(R.alpha);
+    float rho = -0.f;
 #pragma unroll
-    for (int k = 0; k < M; ++k) rho += ffinite_or(U.m[k]);
-    rho = fmaxs(rho, 1e-12f);
-    const float inv_rho = rcp(fmaxs(rho, 1e-10f));
-    float v2 = 0.f;
+    for (int k = 0; k < M; ++k) rho += U.m[k];
+    rho = fmaxs(rho, 1e-10f);
+    const float inv_rho = rcp(rho);
+    float v2 = -0.f;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        R.vel[d] = ffinite_or(U.mom[d]) * inv_rho;
+        R.vel[d] = U.mom[d] * inv_rho;
         v2 += R.vel[d] * R.vel[d];
     }
 #pragma unroll
-    for (int k = 0; k < M; ++k) R.rho[k] = fmaxs(ffinite_or(U.m[k]) * rcp(fmaxs(R.alpha[k], 1e-10f)), 1e-12f);
+    for (int k = 0; k < M; ++k) R.rho[k] = fmaxs(U.m[k] * rcp(fmaxs(R.alpha[k], 1e-10f)), 1e-12f);
     const float kinetic = 0.5f * rho * v2;
-    const float shift = bshift<M>(P, ad, R.alpha);
+    const float shift = P.stiff ? bshift<M>(P, ad, R.alpha) : 0.f;
     float p;
     if (interface_cell<M>(R.alpha)) {
         // FP64 recovery: the energy minus kinetic energy difference, where the
-        // cancellation sits, in FP64 (float products are exact in FP64)
-        double m2 = 0.0;
+        // cancellation sits, in FP64 (float products are exact in FP64); 1/rho
+        // from the FP32 reciprocal and one Newton step (relative error ~1e-14)
+        double m2 = -0.0;
 #pragma unroll
-        for (int d = 0; d < D; ++d) m2 += (double)ffinite_or(U.mom[d]) * (double)ffinite_or(U.mom[d]);
-        const double rho_e_hat = (double)ffinite_or(U.E, kinetic) + (double)shift - 0.5 * m2 / (double)rho;
+        for (int d = 0; d < D; ++d) m2 += (double)U.mom[d] * (double)U.mom[d];
+        const double rd = (double)rho;
+        double inv = (double)inv_rho;
+        inv = inv * (2.0 - rd * inv);
+        const double rho_e_hat = (double)U.E + (double)shift - 0.5 * m2 * inv;
         p = pressure_shifted<M>(P, (float)rho_e_hat, R.alpha);
     } else {
-        p = pressure_shifted<M>(P, ffinite_or(U.E, kinetic) + shift - kinetic, R.alpha);
+        p = pressure_shifted<M>(P, U.E + shift - kinetic, R.alpha);
     }
-    R.p = ffinite(p) ? fmaxs(p, 1e-10f) : 1e-10f;
+    R.p = fmaxs(p, 1e-10f);
 }
 
-// prim_to_cons (primitives.hpp:166-212)
+// prim_to_cons (primitives.hpp:166-212) of a primitive whose fractions are
+// non-negative and sum to one (reconstructed faces after enforce); alpha
+// receives the floored fractions the state was built from
 template<int D, int M>
-__device__ __forceinline__ void prim_to_cons(const EosMixed& P, const Pr<D, M>& Q, St<D, M>& U)
+__device__ __forceinline__ void prim_to_cons(const EosMixed& P, const Pr<D, M>& Q, St<D, M>& U, float* alpha)
 {
-    float alpha[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) alpha[k] = Q.alpha[k];
-    sanitise<M>(alpha, 1e-12f);
-    float rho = 0.f;
+    floor_fractions<M>(alpha);
+    float rho = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) {
         U.m[k] = alpha[k] * fmaxs(Q.rho[k], 1e-12f);
         rho += U.m[k];
     }
     rho = fmaxs(rho, 1e-12f);
-    float v2 = 0.f;
+    float v2 = -0.f;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         U.mom[d] = rho * Q.vel[d];
         v2 += Q.vel[d] * Q.vel[d];
     }
     encode_alpha<M>(alpha, U.a);
-    float ad[M];
-    decode_alpha<M>(U.a, ad);
-    U.E = energy_shifted<M>(P, fmaxs(Q.p, 1e-12f), alpha) + bshift<M>(P, alpha, ad) + 0.5f * rho * v2;
+    float shift = -0.f;
+    if (P.stiff) {
+        float ad[M];
+        decode_alpha<M>(U.a, ad);
+        shift = bshift<M>(P, alpha, ad);
+    }
+    U.E = energy_shifted<M>(P, fmaxs(Q.p, 1e-12f), alpha) + shift + 0.5f * rho * v2;
 }
 
 // repair_state (primitives.hpp:326-457) on a state whose partial masses are
-// of type MT (float for face/half states, double for stored cells)
+// of type MT (float for face/half states, double for stored cells).  The
+// common case (valid fraction code, no inactive mass) is the energy floor
+// alone; fraction renormalisation and the removal of inactive mass with its
+// pressure-preserving rebuild run only when needed.
 template<int D, int M, typename S>
 __device__ __forceinline__ void repair(const EosMixed& P, S& U)
 {
     using MT = decltype(U.m[0] + 0.f);
+    MT m_in[M];
 #pragma unroll
-    for (int k = 0; k < M; ++k) U.m[k] = ffinite(U.m[k]) ? (U.m[k] < MT(0) ? MT(0) : U.m[k]) : MT(0);
-#pragma unroll
-    for (int k = 0; k < M - 1; ++k) U.a[k] = ffinite_or(U.a[k]);
-#pragma unroll
-    for (int d = 0; d < D; ++d) U.mom[d] = ffinite_or(U.mom[d]);
-    U.E = ffinite_or(U.E);
-
-    MT rb = MT(0);
-#pragma unroll
-    for (int k = 0; k < M; ++k) rb += U.m[k];
-    const float rho_before = fmaxs((float)rb, 1e-12f);
-    const float inv_rb = rcp(fmaxs(rho_before, 1e-10f));
-    float vel_before[D];
-    float v2b = 0.f;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        vel_before[d] = U.mom[d] * inv_rb;
-        v2b += vel_before[d] * vel_before[d];
+    for (int k = 0; k < M; ++k) {
+        // negative or NaN mass -> 0
+        U.m[k] = (U.m[k] > MT(0)) ? U.m[k] : MT(0);
+        m_in[k] = U.m[k];
     }
-    float ad0[M], alpha[M], ad1[M];
+    float ad0[M], alpha[M];
     decode_alpha<M>(U.a, ad0);
 #pragma unroll
     for (int k = 0; k < M; ++k) alpha[k] = ad0[k];
-    sanitise<M>(alpha, 0.f);
-    encode_alpha<M>(alpha, U.a);
-    decode_alpha<M>(U.a, ad1);
-    // E_hat now refers to the re-encoded fractions
-    const float e_hat_before = U.E + bshift<M>(P, ad0, alpha);  // rho e - B(alpha) plus kinetic
-    U.E += bshift<M>(P, ad0, ad1);
+    float fix = -0.f;           // B(alpha) - B(stored code): floors refer to the true energy
+    float e_hat_before = U.E;  // rho e - B(alpha) + kinetic, before any change
+    if (!valid_code<M>(U.a)) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) U.mom[d] = ffinite_or(U.mom[d]);
+        U.E = ffinite_or(U.E);
+        sanitise<M>(alpha, 0.f);
+        encode_alpha<M>(alpha, U.a);
+        if (P.stiff) {
+            float ad1[M];
+            decode_alpha<M>(U.a, ad1);
+            e_hat_before = U.E + bshift<M>(P, ad0, alpha);
+            U.E += bshift<M>(P, ad0, ad1);
+            fix = bshift<M>(P, alpha, ad1);
+        } else {
+            e_hat_before = U.E;
+        }
+    }
     bool removed = false;
 #pragma unroll
     for (int k = 0; k < M; ++k)
@@ -332,21 +372,31 @@ __device__ __forceinline__ void repair(const EosMixed& P, S& U)
         rt = MT(1e-12);
     }
     const float rho = (float)rt;
-    const float fix = bshift<M>(P, alpha, ad1);
     if (removed) {
-        const float p_before = fmaxs(pressure_shifted<M>(P, e_hat_before - 0.5f * rho_before * v2b, alpha), 1e-10f);
+        // rebuild momentum and energy at the velocity and pressure before removal
+        MT rb = MT(0);
 #pragma unroll
-        for (int d = 0; d < D; ++d) U.mom[d] = rho * vel_before[d];
+        for (int k = 0; k < M; ++k) rb += m_in[k];
+        const float rho_before = fmaxs((float)rb, 1e-12f);
+        const float inv_rb = rcp(fmaxs(rho_before, 1e-10f));
+        float v2b = -0.f;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const float v = U.mom[d] * inv_rb;
+            U.mom[d] = rho * v;
+            v2b += v * v;
+        }
+        const float p_before = fmaxs(pressure_shifted<M>(P, e_hat_before - 0.5f * rho_before * v2b, alpha), 1e-10f);
         U.E = energy_shifted<M>(P, p_before, alpha) + fix + 0.5f * rho * v2b;
     }
     const float inv = rcp(fmaxs(rho, 1e-10f));
-    float v2 = 0.f;
+    float v2 = -0.f;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         const float v = U.mom[d] * inv;
         v2 += v * v;
     }
-    float emin = 0.f;
+    float emin = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) emin += alpha[k] * P.bmin[k];
     U.E = fmaxs(U.E, 0.5f * rho * v2 + emin + fix);
@@ -362,11 +412,9 @@ __device__ __forceinline__ void enforce(Pr<D, M>& Q)
     Q.p = fmaxs(Q.p, 1e-12f);
 }
 
-__device__ __forceinline__ float minmod(float a, float b)
-{
-    if (a * b <= 0.f) return 0.f;
-    return __builtin_fabsf(a) < __builtin_fabsf(b) ? a : b;
-}
+// MinBee (limiters.hpp:9-15): 0 for opposite signs, else the smaller
+// magnitude, which is the median of (a, b, 0): one v_med3_f32
+__device__ __forceinline__ float minmod(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, 0.f); }
 
 // muscl.hpp:43-112
 template<int D, int M>
@@ -404,10 +452,10 @@ template<int M>
 __device__ __forceinline__ float compressibility(const EosMixed& P, float p, const float* alpha, float* imp,
                                                  bool* active, float floor_alpha)
 {
-    float beta = 0.f;
+    float beta = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-        const float a = fmaxs(ffinite_or(alpha[k]), 0.f);
+        const float a = fmaxs(alpha[k], 0.f);
         imp[k] = fmaxs(P.gamma[k] * (p + P.pinf[k]), 1e-12f);
         active[k] = !(a <= floor_alpha);
         if (active[k]) beta += a * rcp(imp[k]);
@@ -428,18 +476,18 @@ __device__ __forceinline__ float sound_speed(const EosMixed& P, float rho, float
         const float beta = compressibility<M>(P, p, alpha, imp, act, -1.f);
         return frsq(rho * fmaxs(beta, 1e-14f));
     }
-    float cs[M], xi_mix = 0.f, inv_sum = 0.f;
+    float cs[M], xi_mix = -0.f, inv_sum = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-        const float a = fmaxs(ffinite_or(alpha[k]), 0.f);
+        const float a = fmaxs(alpha[k], 0.f);
         cs[k] = fsqrt(P.gamma[k] * (p + P.pinf[k]) * rcp(fmaxs(rho_k[k], 1e-10f)));
         xi_mix += a * P.inv_gm1[k];
         inv_sum += a * rcp(fmaxs(cs[k], 1e-14f));
     }
-    float rc2 = 0.f;
+    float rc2 = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-        const float a = fmaxs(ffinite_or(alpha[k]), 0.f);
+        const float a = fmaxs(alpha[k], 0.f);
         const float lam = (P.lambda_model == LAMBDA_EQUAL_VELOCITY) ? rcp(fmaxs(cs[k], 1e-14f) * inv_sum) : 1.f;
         rc2 += lam * a * P.inv_gm1[k] * cs[k] * cs[k];
     }
@@ -447,21 +495,22 @@ __device__ __forceinline__ float sound_speed(const EosMixed& P, float rho, float
 }
 
 // alpha_rhs_coefficients (primitives.hpp:216-322)
+// Q comes from cons_to_prim: its fractions are sanitised already, so the
+// reference's second sanitise (floor 0) is the identity here.
 template<int D, int M>
 __device__ __forceinline__ void alpha_rhs(const EosMixed& P, const Pr<D, M>& Q, float* rhs)
 {
 #pragma unroll
     for (int k = 0; k < M; ++k) rhs[k] = Q.alpha[k];
-    sanitise<M>(rhs, 0.f);
     if (P.lambda_model == LAMBDA_ALLAIRE) return;
-    const float p = fmaxs(ffinite_or(Q.p), 1e-12f);
+    const float p = fmaxs(Q.p, 1e-12f);
     const float srcf = fmaxs(P.alpha_source_floor, 0.f);
     if (P.lambda_model == LAMBDA_EQUAL_VELOCITY) {
-        float cs[M], inv_sum = 0.f;
+        float cs[M], inv_sum = -0.f;
         bool act[M];
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-            const float a = fmaxs(ffinite_or(rhs[k]), 0.f);
+            const float a = rhs[k];
             act[k] = !(a <= 1e-12f || a < srcf);
             cs[k] = act[k] ? fsqrt(P.gamma[k] * (p + P.pinf[k]) * rcp(fmaxs(Q.rho[k], 1e-12f))) : 1e-12f;
             if (act[k]) inv_sum += a * rcp(cs[k]);
@@ -469,17 +518,17 @@ __device__ __forceinline__ void alpha_rhs(const EosMixed& P, const Pr<D, M>& Q, 
         if (inv_sum <= 1e-12f || !ffinite(inv_sum)) return;
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-            const float a = fmaxs(ffinite_or(rhs[k]), 0.f);
+            const float a = rhs[k];
             rhs[k] = (a <= 1e-12f) ? 0.f : a * rcp(cs[k] * inv_sum);
         }
         return;
     }
     float imp[M];
     bool act[M];
-    float beta = 0.f;
+    float beta = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-        const float a = fmaxs(ffinite_or(rhs[k]), 0.f);
+        const float a = rhs[k];
         act[k] = !(a <= 1e-12f || a < srcf);
         imp[k] = fmaxs(P.gamma[k] * (p + P.pinf[k]), 1e-12f);
         if (act[k]) beta += a * rcp(imp[k]);
@@ -488,7 +537,7 @@ __device__ __forceinline__ void alpha_rhs(const EosMixed& P, const Pr<D, M>& Q, 
     const float q = rcp(beta);
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-        const float a = fmaxs(ffinite_or(rhs[k]), 0.f);
+        const float a = rhs[k];
         rhs[k] = act[k] ? a + a * (q * rcp(imp[k]) - 1.f) : 0.f;
     }
 }
@@ -497,9 +546,7 @@ __device__ __forceinline__ void alpha_rhs(const EosMixed& P, const Pr<D, M>& Q, 
 template<int D, int M, int DIR>
 __device__ __forceinline__ void flux_normal(const EosMixed& P, const St<D, M>& U, const Pr<D, M>& Q, Fx<D, M>& F)
 {
-    float ad[M];
-    decode_alpha<M>(U.a, ad);
-    float rho = 0.f;
+    float rho = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) rho += U.m[k];
     const float un = Q.vel[DIR];
@@ -507,7 +554,13 @@ __device__ __forceinline__ void flux_normal(const EosMixed& P, const St<D, M>& U
     for (int k = 0; k < M; ++k) F.m[k] = U.m[k] * un;
 #pragma unroll
     for (int d = 0; d < D; ++d) F.mom[d] = rho * Q.vel[d] * un + (d == DIR ? Q.p : 0.f);
-    F.E = (U.E + bdot<M>(P, ad) + Q.p) * un;
+    float B = -0.f;
+    if (P.stiff) {
+        float ad[M];
+        decode_alpha<M>(U.a, ad);
+        B = bdot<M>(P, ad);
+    }
+    F.E = (U.E + B + Q.p) * un;
 }
 
 // muscl.hpp:143-191
@@ -534,11 +587,15 @@ __device__ __forceinline__ void hancock(const EosMixed& P, const St<D, M>& Uc, c
     for (int k = 0; k < M; ++k)
         an[k] = Pc.alpha[k] - factor * (rf.alpha[k] * ur_ - lf.alpha[k] * ul_) + 0.5f * dt * rhs[k] * div_u;
     sanitise<M>(an, 0.f);
-    float ad_old[M], ad_new[M];
-    decode_alpha<M>(out.a, ad_old);
-    encode_alpha<M>(an, out.a);
-    decode_alpha<M>(out.a, ad_new);
-    out.E += bshift<M>(P, ad_old, ad_new);
+    if (P.stiff) {
+        float ad_old[M], ad_new[M];
+        decode_alpha<M>(out.a, ad_old);
+        encode_alpha<M>(an, out.a);
+        decode_alpha<M>(out.a, ad_new);
+        out.E += bshift<M>(P, ad_old, ad_new);
+    } else {
+        encode_alpha<M>(an, out.a);
+    }
     repair<D, M>(P, out);
 }
 
@@ -547,7 +604,7 @@ template<int D, int M, int DIR>
 __device__ __forceinline__ bool hllc(const EosMixed& P, const St<D, M>& UL, const St<D, M>& UR, const Pr<D, M>& PL,
                                      const Pr<D, M>& PR, Fx<D, M>& F, float& fv)
 {
-    float rhoL = 0.f, rhoR = 0.f;
+    float rhoL = -0.f, rhoR = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) {
         rhoL += UL.m[k];
@@ -602,9 +659,13 @@ __device__ __forceinline__ bool hllc(const EosMixed& P, const St<D, M>& UL, cons
     // star state (hllc.hpp:15-41, 144-194) written through
     // delta = rho*/rho_K - 1 = (s* - u_K)/(s_K - s*) so that E* - E_K carries
     // no cancellation against the stiffened-gas energy
-    float adK[M];
-    decode_alpha<M>(UK.a, adK);
-    const float EK = UK.E + bdot<M>(P, adK);
+    float BK = -0.f;
+    if (P.stiff) {
+        float adK[M];
+        decode_alpha<M>(UK.a, adK);
+        BK = bdot<M>(P, adK);
+    }
+    const float EK = UK.E + BK;
     const float ratio = safe_div_f(sK - unK, sK - s_star);
     const float rho_star = rhoK * ratio;
     const float delta = safe_div_f(s_star - unK, sK - s_star);
@@ -612,9 +673,13 @@ __device__ __forceinline__ bool hllc(const EosMixed& P, const St<D, M>& UL, cons
     const float E_star = rho_star * safe_div_f(EK, rhoK) + e_star_term;
     fv = s_star;
     if (rho_star <= 0.f || !ffinite(rho_star) || !ffinite(E_star)) {
-        float adL[M], adR[M];
-        decode_alpha<M>(UL.a, adL);
-        decode_alpha<M>(UR.a, adR);
+        float dB = -0.f;
+        if (P.stiff) {
+            float adL[M], adR[M];
+            decode_alpha<M>(UL.a, adL);
+            decode_alpha<M>(UR.a, adR);
+            dB = bshift<M>(P, adR, adL);
+        }
         const float inv = rcp(sR - sL);
         const float sLsR = sL * sR;
 #pragma unroll
@@ -622,7 +687,7 @@ __device__ __forceinline__ bool hllc(const EosMixed& P, const St<D, M>& UL, cons
 #pragma unroll
         for (int d = 0; d < D; ++d)
             F.mom[d] = inv * ((sR * FL.mom[d] - sL * FR.mom[d]) + sLsR * (UR.mom[d] - UL.mom[d]));
-        F.E = inv * ((sR * FL.E - sL * FR.E) + sLsR * ((UR.E - UL.E) + bshift<M>(P, adR, adL)));
+        F.E = inv * ((sR * FL.E - sL * FR.E) + sLsR * ((UR.E - UL.E) + dB));
         return true;
     }
 #pragma unroll
@@ -690,7 +755,7 @@ struct Fp32Mixed {
         for (int d = 0; d < D; ++d) U.mom[d] = (float)p[o++];
         const double E = p[o++];
         double al[M];
-        double sum = 0.0;
+        double sum = -0.0;
 #pragma unroll
         for (int k = 0; k < M - 1; ++k) {
             al[k] = p[o++];
@@ -717,7 +782,7 @@ struct Fp32Mixed {
             al[0] = pos ? (double)a[0] : 1.0 + (double)a[0];
             al[1] = pos ? 1.0 - (double)a[0] : -(double)a[0];
         } else {
-            double sum = 0.0;
+            double sum = -0.0;
 #pragma unroll
             for (int k = 0; k < M - 1; ++k) {
                 al[k] = a[k];
@@ -725,7 +790,7 @@ struct Fp32Mixed {
             }
             al[M - 1] = 1.0 - sum;
         }
-        double b = 0.0;
+        double b = -0.0;
 #pragma unroll
         for (int k = 0; k < M; ++k) b += P.b_d[k] * al[k];
         return b;
@@ -794,7 +859,8 @@ struct Fp32Mixed {
     template<int D, int M>
     static __device__ __forceinline__ void prim_to_cons(const Eos& P, const Pr<D, M>& Q, St<D, M>& U)
     {
-        f32::prim_to_cons<D, M>(P, Q, U);
+        float alpha[M];
+        f32::prim_to_cons<D, M>(P, Q, U, alpha);
     }
     template<int D, int M>
     static __device__ __forceinline__ void repair(const Eos& P, St<D, M>& U) { f32::repair<D, M>(P, U); }
@@ -804,21 +870,20 @@ struct Fp32Mixed {
     // are non-negative, fractions sanitised and the energy above its floor by
     // construction.  Only then is the full repair run.
     template<int D, int M>
-    static __device__ __forceinline__ bool needs_repair(const St<D, M>& U)
+    static __device__ __forceinline__ bool needs_repair(const St<D, M>& U, const float* alpha)
     {
-        float ad[M];
-        f32::decode_alpha<M>(U.a, ad);
         bool r = false;
 #pragma unroll
-        for (int k = 0; k < M; ++k) r |= (ad[k] <= 1e-10f && U.m[k] > 1e-12f);
+        for (int k = 0; k < M; ++k) r |= (alpha[k] <= 1e-10f && U.m[k] > 1e-12f);
         return r;
     }
 
     template<int D, int M>
     static __device__ __forceinline__ void face_cons(const Eos& P, const Pr<D, M>& Q, St<D, M>& U)
     {
-        f32::prim_to_cons<D, M>(P, Q, U);
-        if (needs_repair<D, M>(U)) f32::repair<D, M>(P, U);
+        float alpha[M];
+        f32::prim_to_cons<D, M>(P, Q, U, alpha);
+        if (needs_repair<D, M>(U, alpha)) f32::repair<D, M>(P, U);
     }
 
     // face state and its primitive: when repair has nothing to do the
@@ -827,16 +892,13 @@ struct Fp32Mixed {
     template<int D, int M>
     static __device__ __forceinline__ void face_state(const Eos& P, const Pr<D, M>& Q, St<D, M>& U, Pr<D, M>& QU)
     {
-        f32::prim_to_cons<D, M>(P, Q, U);
-        if (needs_repair<D, M>(U)) {
+        float alpha[M];
+        f32::prim_to_cons<D, M>(P, Q, U, alpha);
+        if (needs_repair<D, M>(U, alpha)) {
             f32::repair<D, M>(P, U);
             f32::cons_to_prim<D, M>(P, U, QU);
             return;
         }
-        float alpha[M];
-#pragma unroll
-        for (int k = 0; k < M; ++k) alpha[k] = Q.alpha[k];
-        f32::sanitise<M>(alpha, 1e-12f);
 #pragma unroll
         for (int k = 0; k < M; ++k) {
             QU.alpha[k] = alpha[k];
@@ -883,11 +945,15 @@ struct Fp32Mixed {
 #pragma unroll
         for (int k = 0; k < M; ++k) an[k] = Pn.alpha[k] - lf * (afr[k] - afl[k]) + dtf * rhs[k] * div_u;
         f32::sanitise<M>(an, 0.f);
-        float ad_old[M], ad_new[M];
-        f32::decode_alpha<M>(r.a, ad_old);
-        f32::encode_alpha<M>(an, r.a);
-        f32::decode_alpha<M>(r.a, ad_new);
-        r.E += f32::bshift<M>(P, ad_old, ad_new);
+        if (P.stiff) {
+            float ad_old[M], ad_new[M];
+            f32::decode_alpha<M>(r.a, ad_old);
+            f32::encode_alpha<M>(an, r.a);
+            f32::decode_alpha<M>(r.a, ad_new);
+            r.E += f32::bshift<M>(P, ad_old, ad_new);
+        } else {
+            f32::encode_alpha<M>(an, r.a);
+        }
         f32::repair<D, M>(P, r);
     }
 
@@ -927,7 +993,9 @@ struct Fp32Mixed {
     template<int D, int M>
     static __device__ __forceinline__ void repair_cell(const Eos& P, Cell<D, M>& U) { f32::repair<D, M>(P, U); }
 
-    // FP64 candidate cfl*dx/(|u|+c) from the FP32 wave speed
+    // FP64 candidate min_d cfl*dx_d/(|u_d|+c) (cfl.hpp:43-63) from the FP32
+    // wave speed, as 1 / max_d((|u_d|+c) / (cfl*dx_d)): one FP64 division
+    // per cell; equal to the reference's form up to FP64 rounding
     template<int D, int M>
     static __device__ __forceinline__ double cfl_candidate(const Eos& P, const Cell<D, M>& C, const CflArgs& a)
     {
@@ -935,24 +1003,23 @@ struct Fp32Mixed {
         to_state<D, M>(C, U);
         Pr<D, M> Q;
         f32::cons_to_prim<D, M>(P, U, Q);
-        float rho = 0.f;
+        float rho = -0.f;
 #pragma unroll
         for (int k = 0; k < M; ++k) rho += U.m[k];
         const double c = (double)f32::sound_speed<M>(P, rho, Q.p, Q.alpha, Q.rho);
-        double best = bits_to_double(kInfBits);
+        double rate = 0.0;
         if (!a.unsplit) {
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 const double ws = (double)__builtin_fabsf(Q.vel[d]) + c;
-                if (ws > 1e-14) best = smin(best, a.cfl_dx[d] / ws);
+                if (ws > 1e-14) rate = __builtin_fmax(rate, ws * a.inv_cfl_dx[d]);
             }
-        } else {
-            double spectral = 0.0;
-#pragma unroll
-            for (int d = 0; d < D; ++d) spectral += ((double)__builtin_fabsf(Q.vel[d]) + c) / a.dx[d];
-            if (spectral > 1e-14) best = smin(best, a.cfl / spectral);
+            return rate > 0.0 ? 1.0 / rate : bits_to_double(kInfBits);
         }
-        return best;
+        double spectral = -0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) spectral += ((double)__builtin_fabsf(Q.vel[d]) + c) / a.dx[d];
+        return spectral > 1e-14 ? a.cfl / spectral : bits_to_double(kInfBits);
     }
 
     template<int D, int M>

@@ -9,6 +9,7 @@ namespace dimhip {
 
 struct CflArgs {
     double cfl_dx[3];
+    double inv_cfl_dx[3];  // 1 / (cfl * dx): the FP32-mixed form's wave-rate reduction
     double dx[3];
     double cfl;
     int unsplit;
