@@ -120,7 +120,21 @@ template<int M>
 struct MixCoef {
     double coef[M];
     double den[M];
+    double rden[M];  // RN(1/den): the residual's divisions by den become div_rn
 };
+
+// a/b correctly rounded from y = RN(1/b) (Markstein's theorem: q = a*y is
+// within one ulp of a/b, r = a - b*q is exact with an FMA, and RN(q + r*y) is
+// RN(a/b)).  Valid for finite a and normal b with a/b, a*y and r = a - b*q
+// in the normal range, which the bisection's operands satisfy (a >= 1e-10,
+// 1e-10 <= b, a/b < 1e300): three FMA-pipe operations in place of the
+// ten-instruction IEEE division sequence, with the same bits.
+__device__ __forceinline__ double div_rn(double a, double b, double y)
+{
+    const double q = a * y;
+    const double r = __builtin_fma(-b, q, a);
+    return __builtin_fma(r, y, q);
+}
 
 template<int M>
 __device__ __forceinline__ MixCoef<M> mix_coef(const EosParams& P, const double* alpha, const double* rho)
@@ -131,6 +145,7 @@ __device__ __forceinline__ MixCoef<M> mix_coef(const EosParams& P, const double*
         const double rk = clamp_min(rho[k]);
         c.coef[k] = smax(finite_or(alpha[k]), 0.0) * rk;
         c.den[k] = safe_denom(P.gm1[k] * rk);
+        c.rden[k] = 1.0 / c.den[k];
     }
     return c;
 }
@@ -144,7 +159,7 @@ __device__ __forceinline__ double mix_energy(const EosParams& P, const MixCoef<M
 #pragma unroll
     for (int k = 0; k < M; ++k) {
         const double num = (P.kind[k] == EOS_STIFFENED) ? (pk + P.gpinf[k]) : pk;
-        s += c.coef[k] * (finite_or(num) / c.den[k]);
+        s += c.coef[k] * div_rn(finite_or(num), c.den[k], c.rden[k]);
     }
     return finite_or(s);
 }
@@ -201,7 +216,15 @@ __device__ __forceinline__ double internal_energy_from_pressure(const EosParams&
     rho_total = clamp_min(rho_total);
     p = clamp_min(p);
     const MixCoef<M> c = mix_coef<M>(P, alpha, rho);
-    return mix_energy<M>(P, c, p) / rho_total;
+    // one evaluation: the plain division (same bits as div_rn, fewer operations here)
+    const double pk = clamp_min(p);
+    double sum = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const double num = (P.kind[k] == EOS_STIFFENED) ? (pk + P.gpinf[k]) : pk;
+        sum += c.coef[k] * (finite_or(num) / c.den[k]);
+    }
+    return finite_or(sum) / rho_total;
 }
 
 // eos.hpp:195-221 (Wood) and eos.hpp:223-275 (allaire / equal_velocity)

@@ -90,6 +90,10 @@ struct Cell {
     float a[M > 1 ? M - 1 : 1];
 };
 
+// sanitise_alpha (primitives.hpp:45-70).  Fractions that already sum to
+// within 1e-4 of one are renormalised with 1 - (sum - 1), whose error
+// (sum - 1)^2 <= 1e-8 is below FP32 resolution; other sums take the
+// reciprocal.
 template<int M>
 __device__ __forceinline__ void sanitise(float* alpha, float floor)
 {
@@ -99,12 +103,16 @@ __device__ __forceinline__ void sanitise(float* alpha, float floor)
         alpha[k] = fmaxs(alpha[k], floor);
         sum += alpha[k];
     }
-    if (!ffinite(sum) || sum <= 0.f) {
+    const float dev = sum - 1.f;
+    float inv = 1.f - dev;
+    if (!(__builtin_fabsf(dev) <= 1e-4f)) {
+        if (!ffinite(sum) || sum <= 0.f) {
 #pragma unroll
-        for (int k = 0; k < M; ++k) alpha[k] = (k == 0) ? 1.f : 0.f;
-        return;
+            for (int k = 0; k < M; ++k) alpha[k] = (k == 0) ? 1.f : 0.f;
+            return;
+        }
+        inv = rcp(sum);
     }
-    const float inv = rcp(sum);
 #pragma unroll
     for (int k = 0; k < M; ++k) alpha[k] *= inv;
 }
@@ -471,10 +479,21 @@ __device__ __forceinline__ float sound_speed(const EosMixed& P, float rho, float
     rho = fmaxs(rho, 1e-10f);
     p = fmaxs(p, 1e-10f);
     if (P.lambda_model == LAMBDA_KAPILA) {
-        float imp[M];
-        bool act[M];
-        const float beta = compressibility<M>(P, p, alpha, imp, act, -1.f);
-        return frsq(rho * fmaxs(beta, 1e-14f));
+        if constexpr (M == 2) {
+            // c^2 = 1/(rho (a0/I0 + a1/I1)) = Y/(rho X), X = a0 I1 + a1 I0, Y = I0 I1,
+            // so c = Y rsq(rho X Y): one transcendental
+            const float I0 = fmaxs(P.gamma[0] * (p + P.pinf[0]), 1e-12f);
+            const float I1 = fmaxs(P.gamma[1] * (p + P.pinf[1]), 1e-12f);
+            const float a0 = fmaxs(alpha[0], 0.f), a1 = fmaxs(alpha[1], 0.f);
+            const float X = fmaxs(a0 * I1 + a1 * I0, 1e-30f);
+            const float Y = I0 * I1;
+            return Y * frsq(rho * X * Y);
+        } else {
+            float imp[M];
+            bool act[M];
+            const float beta = compressibility<M>(P, p, alpha, imp, act, -1.f);
+            return frsq(rho * fmaxs(beta, 1e-14f));
+        }
     }
     float cs[M], xi_mix = -0.f, inv_sum = -0.f;
 #pragma unroll
@@ -522,6 +541,19 @@ __device__ __forceinline__ void alpha_rhs(const EosMixed& P, const Pr<D, M>& Q, 
             rhs[k] = (a <= 1e-12f) ? 0.f : a * rcp(cs[k] * inv_sum);
         }
         return;
+    }
+    if constexpr (M == 2) {
+        // both phases active: q/I0 = I1/X and q/I1 = I0/X with X = a0 I1 + a1 I0,
+        // one reciprocal for the pair
+        const float a0 = rhs[0], a1 = rhs[1];
+        if (!(a0 <= 1e-12f || a0 < srcf) && !(a1 <= 1e-12f || a1 < srcf)) {
+            const float I0 = fmaxs(P.gamma[0] * (p + P.pinf[0]), 1e-12f);
+            const float I1 = fmaxs(P.gamma[1] * (p + P.pinf[1]), 1e-12f);
+            const float invX = rcp(a0 * I1 + a1 * I0);
+            rhs[0] = a0 + a0 * (I1 * invX - 1.f);
+            rhs[1] = a1 + a1 * (I0 * invX - 1.f);
+            return;
+        }
     }
     float imp[M];
     bool act[M];
@@ -666,9 +698,10 @@ __device__ __forceinline__ bool hllc(const EosMixed& P, const St<D, M>& UL, cons
         BK = bdot<M>(P, adK);
     }
     const float EK = UK.E + BK;
-    const float ratio = safe_div_f(sK - unK, sK - s_star);
+    const float inv_gap = rcp(safe_denom_f(sK - s_star));
+    const float ratio = (sK - unK) * inv_gap;
     const float rho_star = rhoK * ratio;
-    const float delta = safe_div_f(s_star - unK, sK - s_star);
+    const float delta = (s_star - unK) * inv_gap;
     const float e_star_term = rho_star * (s_star - unK) * (s_star + safe_div_f(pK, rhoK * (sK - unK)));
     const float E_star = rho_star * safe_div_f(EK, rhoK) + e_star_term;
     fv = s_star;
@@ -704,6 +737,7 @@ __device__ __forceinline__ bool hllc(const EosMixed& P, const St<D, M>& UL, cons
 }  // namespace f32
 
 struct Fp32Mixed {
+    static constexpr int kMinWavesPerEU = 1;
     using R = float;
     using Eos = EosMixed;
     template<int D, int M> using St = f32::St<D, M>;

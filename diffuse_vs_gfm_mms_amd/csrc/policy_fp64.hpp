@@ -20,6 +20,7 @@ __device__ __forceinline__ double bits_to_double(unsigned long long b) { return 
 __device__ __forceinline__ unsigned long long double_to_bits(double x) { return (unsigned long long)__double_as_longlong(x); }
 
 struct Fp64 {
+    static constexpr int kMinWavesPerEU = 4;
     using R = double;
     using Eos = EosParams;
     template<int D, int M> using St = State<D, M>;
