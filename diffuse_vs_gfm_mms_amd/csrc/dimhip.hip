@@ -14,6 +14,7 @@
 
 #include "../../include/dimhip.h"
 #include "batch.hpp"
+#include "ic_kernels.hpp"
 #include "io_kernels.hpp"
 #include "kernels_fp64.hpp"
 #include "policy_fp32.hpp"
@@ -146,6 +147,8 @@ struct Ops {
     virtual void prim_to_packed(dimhip_ctx* c, const double* prim, double* packed, long long count) const = 0;
     virtual void output_rows(dimhip_ctx* c, const double* packed, double* rows, long long count) const = 0;
     virtual void boundary_flux(dimhip_ctx* c, const void* q, double* integral) const = 0;
+    virtual void ic_prims(dimhip_ctx* c, const IcDesc& ic, long long first, long long count, double* prim,
+                          int* err) const = 0;
 };
 
 CflArgs make_cfl_args(const dimhip_ctx* c, double cfl, bool unsplit)
@@ -265,6 +268,13 @@ struct OpsImpl final : Ops {
     void output_rows(dimhip_ctx* c, const double* packed, double* rows, long long count) const override
     {
         output_rows_kernel<D, M><<<grid_blocks(count, 256), 256, 0, c->stream>>>(c->eos, packed, rows, count);
+    }
+
+    void ic_prims(dimhip_ctx* c, const IcDesc& ic, long long first, long long count, double* prim,
+                  int* err) const override
+    {
+        ic_prim_kernel<D, M><<<grid_blocks(count, 256), 256, 0, c->stream>>>(ic, c->grid.N[0], c->grid.N[1],
+                                                                             c->grid.N[2], first, count, prim, err);
     }
 
     void boundary_flux(dimhip_ctx* c, const void* q, double* integral) const override
@@ -821,6 +831,90 @@ int dimhip_upload_primitive(dimhip_ctx* c, const double* prim, size_t ncells)
         HIP_OK(c, hipStreamSynchronize(c->stream));
     }
     c->dt_ready = false;
+    return 0;
+}
+
+int dimhip_initialise(dimhip_ctx* c, const dimhip_ic* ic)
+{
+    if (!c || !ic) return fail(c, "dimhip_initialise: null argument");
+    const int D = c->desc.dim, M = c->desc.nmat;
+    if (ic->kind < DIMHIP_IC_REGIONS || ic->kind > DIMHIP_IC_COATED_SHOCK_BUBBLE)
+        return fail(c, "dim::initialise_dim_from_config: unsupported initial_condition");
+    IcDesc d{};
+    d.kind = ic->kind;
+    for (int a = 0; a < 3; ++a) {
+        d.domain_min[a] = ic->domain_min[a];
+        d.domain_max[a] = ic->domain_max[a];
+        d.dx[a] = a < D ? (ic->domain_max[a] - ic->domain_min[a]) / static_cast<double>(c->grid.N[a]) : 1.0;
+        d.centre[a] = ic->bubble_center[a];
+    }
+    d.thickness = ic->interface_thickness;
+    d.sharpness = ic->sharpness_alpha;
+    const bool region_ic = ic->kind == DIMHIP_IC_REGIONS || ic->kind == DIMHIP_IC_PLANAR_REGIONS;
+    if (region_ic) {
+        if (ic->nregions <= 0) return fail(c, "dim::diffuse_region_primitive: no regions defined");
+        if (ic->nregions > DIMHIP_MAX_REGIONS) return fail(c, "dimhip_initialise: too many regions");
+        d.nregions = ic->nregions;
+        for (int i = 0; i < ic->nregions; ++i) {
+            const dimhip_ic_region& r = ic->regions[i];
+            if (r.material < 0 || r.material >= M)
+                return fail(c, "dim::material_slot_from_id: material id not found");
+            IcRegion& o = d.regions[i];
+            for (int a = 0; a < 3; ++a) {
+                o.lower[a] = r.lower[a];
+                o.upper[a] = r.upper[a];
+                o.vel[a] = r.vel[a];
+            }
+            o.rho = r.rho;
+            o.p = r.p;
+            o.slot = r.material;
+        }
+    }
+    if (ic->kind == DIMHIP_IC_PLANAR_REGIONS) {
+        // normalised_planar_normal (dim_initial_conditions.hpp:33-55)
+        double n2 = 0.0;
+        for (int a = 0; a < D; ++a) n2 += ic->planar_normal[a] * ic->planar_normal[a];
+        if (n2 <= 0.0) return fail(c, "dim::normalised_planar_normal: zero planar normal");
+        const double inv = 1.0 / std::sqrt(n2);
+        for (int a = 0; a < D; ++a) d.normal[a] = ic->planar_normal[a] * inv;
+    }
+    if (!region_ic) {
+        if (ic->shock_axis < 0 || ic->shock_axis >= D) return fail(c, "Invalid shock axis");
+        if (!(ic->bubble_radius > 0.0)) return fail(c, "Invalid bubble radius");
+        const dimhip_ic_state* src[4] = {&ic->left, &ic->right, &ic->bubble, &ic->film};
+        IcPhase* dst[4] = {&d.left, &d.right, &d.bubble, &d.film};
+        const int used = ic->kind == DIMHIP_IC_COATED_SHOCK_BUBBLE ? 4 : 3;
+        for (int i = 0; i < 4; ++i) {
+            if (i < used && (src[i]->material < 0 || src[i]->material >= M))
+                return fail(c, "dim::material_slot_from_id: material id not found");
+            dst[i]->rho = src[i]->rho;
+            dst[i]->p = src[i]->p;
+            dst[i]->slot = src[i]->material;
+            for (int a = 0; a < 3; ++a) dst[i]->vel[a] = src[i]->vel[a];
+        }
+        d.shock_axis = ic->shock_axis;
+        d.shock_position = ic->shock_position;
+        d.radius = ic->bubble_radius;
+        d.film_radius = ic->film_radius;
+    }
+    HIP_OK(c, hipSetDevice(c->desc.device));
+    const int np = c->nv + 1;
+    const long long chunk = std::min<long long>(c->grid.ncells, 1ll << 22);
+    if (ensure_staging(c, chunk) || ensure_io(c, (size_t)chunk * np + 1)) return 1;
+    int* derr = (int*)(c->io_buf + (size_t)chunk * np);
+    HIP_OK(c, hipMemsetAsync(derr, 0, sizeof(int), c->stream));
+    for (long long first = 0; first < c->grid.ncells; first += chunk) {
+        const long long cnt = std::min(chunk, c->grid.ncells - first);
+        c->ops->ic_prims(c, d, first, cnt, c->io_buf, derr);
+        c->ops->prim_to_packed(c, c->io_buf, c->staging, cnt);
+        c->ops->pack(c, true, c->staging, c->q[c->cur], first, cnt);
+        HIP_OK(c, hipGetLastError());
+    }
+    int herr = 0;
+    HIP_OK(c, hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    c->dt_ready = false;
+    if (herr & IC_ERR_UNCOVERED) return fail(c, "dim::sharp_region_primitive: point not covered by any region");
     return 0;
 }
 

@@ -48,6 +48,83 @@ class Desc(C.Structure):
     ]
 
 
+IC_KIND = {"regions": 0, "planar_regions": 1, "shock_bubble": 2, "coated_shock_bubble": 3}
+MAX_REGIONS = 16
+
+
+class IcRegion(C.Structure):
+    _fields_ = [("lower", C.c_double * 3), ("upper", C.c_double * 3), ("rho", C.c_double),
+                ("vel", C.c_double * 3), ("p", C.c_double), ("material", C.c_int)]
+
+
+class IcState(C.Structure):
+    _fields_ = [("rho", C.c_double), ("vel", C.c_double * 3), ("p", C.c_double), ("material", C.c_int)]
+
+
+class Ic(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("domain_min", C.c_double * 3),
+        ("domain_max", C.c_double * 3),
+        ("interface_thickness", C.c_double),
+        ("sharpness_alpha", C.c_double),
+        ("planar_normal", C.c_double * 3),
+        ("nregions", C.c_int),
+        ("regions", IcRegion * MAX_REGIONS),
+        ("shock_axis", C.c_int),
+        ("shock_position", C.c_double),
+        ("left", IcState),
+        ("right", IcState),
+        ("bubble", IcState),
+        ("film", IcState),
+        ("bubble_center", C.c_double * 3),
+        ("bubble_radius", C.c_double),
+        ("film_radius", C.c_double),
+    ]
+
+
+def _fill_state(dst, st):
+    rho, vel, p, slot = st
+    dst.rho, dst.p, dst.material = rho, p, slot
+    for a, v in enumerate(vel):
+        dst.vel[a] = v
+
+
+def make_ic(kind, domain_min, domain_max, thickness, sharpness=2.0, regions=(), planar_normal=None,
+            shock_axis=0, shock_position=0.0, left=None, right=None, bubble=None, film=None,
+            centre=(0.0, 0.0, 0.0), radius=0.0, film_radius=0.0):
+    """An initial-condition description for Solver.initialise.  States are
+    (rho, vel tuple, p, material slot); regions are (lower, upper, rho, vel, p, slot)."""
+    ic = Ic()
+    ic.kind = IC_KIND[kind]
+    for a in range(len(domain_min)):
+        ic.domain_min[a], ic.domain_max[a] = domain_min[a], domain_max[a]
+    ic.interface_thickness, ic.sharpness_alpha = thickness, sharpness
+    if planar_normal is not None:
+        for a, v in enumerate(planar_normal):
+            ic.planar_normal[a] = v
+    if len(regions) > MAX_REGIONS:
+        raise RuntimeError("dimhip_initialise: too many regions")
+    ic.nregions = len(regions)
+    for i, (lower, upper, rho, vel, p, slot) in enumerate(regions):
+        r = ic.regions[i]
+        for a in range(len(lower)):
+            r.lower[a], r.upper[a] = lower[a], upper[a]
+        for a, v in enumerate(vel):
+            r.vel[a] = v
+        r.rho, r.p, r.material = rho, p, slot
+    ic.shock_axis, ic.shock_position = shock_axis, shock_position
+    for name, st in (("left", left), ("right", right), ("bubble", bubble), ("film", film)):
+        if st is not None:
+            _fill_state(getattr(ic, name), st)
+        else:
+            getattr(ic, name).material = -1
+    for a, v in enumerate(centre):
+        ic.bubble_center[a] = v
+    ic.bubble_radius, ic.film_radius = radius, film_radius
+    return ic
+
+
 class Timings(C.Structure):
     _fields_ = [("cfl_seconds", C.c_double), ("sweep_seconds", C.c_double),
                 ("boundary_seconds", C.c_double)]
@@ -81,6 +158,7 @@ def load_library():
         "dimhip_track_boundary_flux": (C.c_int, [vp, C.c_int]),
         "dimhip_boundary_flux_integral": (C.c_int, [vp, dp]),
         "dimhip_upload_primitive": (C.c_int, [vp, dp, C.c_size_t]),
+        "dimhip_initialise": (C.c_int, [vp, C.POINTER(Ic)]),
         "dimhip_output_rows": (C.c_int, [vp, dp, C.c_size_t]),
         "dimhip_synchronize": (C.c_int, [vp]),
         "dimhip_set_timing": (C.c_int, [vp, C.c_int]),
@@ -110,7 +188,7 @@ def exported_symbols():
     return ["dimhip_version", "dimhip_create", "dimhip_destroy", "dimhip_upload", "dimhip_download",
             "dimhip_step", "dimhip_advance", "dimhip_compute_dt", "dimhip_sweep", "dimhip_apply_bc",
             "dimhip_conservation", "dimhip_boundary_flux", "dimhip_track_boundary_flux",
-            "dimhip_boundary_flux_integral", "dimhip_upload_primitive", "dimhip_output_rows", "dimhip_synchronize", "dimhip_set_timing", "dimhip_kernel_times",
+            "dimhip_boundary_flux_integral", "dimhip_upload_primitive", "dimhip_initialise", "dimhip_output_rows", "dimhip_synchronize", "dimhip_set_timing", "dimhip_kernel_times",
             "dimhip_kat", "dimhip_last_error", "dimhip_batch_create", "dimhip_batch_destroy",
             "dimhip_batch_upload", "dimhip_batch_download", "dimhip_batch_configure", "dimhip_batch_autotune",
             "dimhip_batch_advance", "dimhip_batch_set_timing", "dimhip_batch_kernel_times",
@@ -161,10 +239,22 @@ class Solver:
 
     @classmethod
     def from_case(cls, case, precision="fp64", time_update="split", **kw):
+        """A solver holding the case's initial state: its packed U when the case
+        carries one, else its initial condition generated on the device."""
         s = cls(case.dim, case.N, case.dx, case.materials, case.bc_lo, case.bc_hi,
                 case.lambda_model, case.alpha_source_floor, time_update, precision, **kw)
-        s.upload(case.U)
+        if case.U is not None:
+            s.upload(case.U)
+        else:
+            s.initialise(case.ic)
         return s
+
+    def initialise(self, ic):
+        """Initial condition generated on the device (dimhip_initialise); ic is
+        an Ic from make_ic or a dict of make_ic's keyword arguments."""
+        if isinstance(ic, dict):
+            ic = make_ic(**ic)
+        self._check(self.lib.dimhip_initialise(self.h, C.byref(ic)))
 
     def _check(self, rc, h="self"):
         if rc != 0:

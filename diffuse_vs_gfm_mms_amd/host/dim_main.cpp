@@ -1,6 +1,6 @@
 // dim_main.cpp — DIM case driver on the GPU library.
 //
-//     dim_main_<D>d <config_file> [--precision fp64|fp32_mixed] [--graph]
+//     dim_main_<D>d <config_file> [--precision fp64|fp32_mixed] [--graph] [--device-ic]
 //
 // The reference's mm_main (src/app/multimaterial_main.cpp:17-62) for
 // `interface_method = DIM` case files: the same command line, case-file format,
@@ -14,6 +14,10 @@
 // The precision form is chosen outside the case file (the format stays the
 // reference's): --precision, or DIMHIP_PRECISION in the environment; the
 // default is the FP64 form, which reproduces the reference bit for bit.
+// --device-ic generates the initial state on the GPU (dimhip_initialise)
+// instead of building primitive rows on the host; its tanh is the device
+// library's, so diffuse initial cells may differ from the reference's in the
+// last bit.
 #include <algorithm>
 #include <array>
 #include <chrono>
@@ -39,6 +43,7 @@ struct Options {
     std::string config;
     int precision = DIMHIP_FP64;
     bool graph = false;
+    bool device_ic = false;
 };
 
 inline void check(int status, const dimhip_ctx* ctx)
@@ -124,6 +129,66 @@ dimhip_desc make_desc(const CaseConfig<DIM>& c, const std::array<int, DIM>& N, i
     return d;
 }
 
+// The case's initial condition for dimhip_initialise (material ids -> slots,
+// material_slot_from_id, dim_initial_conditions.hpp:170-183).
+template<int DIM>
+dimhip_ic make_ic(const CaseConfig<DIM>& c)
+{
+    dimhip_ic ic{};
+    const std::string& k = c.initial_condition;
+    if (k == "regions") ic.kind = DIMHIP_IC_REGIONS;
+    else if (k == "planar_regions") ic.kind = DIMHIP_IC_PLANAR_REGIONS;
+    else if (k == "shock_bubble") ic.kind = DIMHIP_IC_SHOCK_BUBBLE;
+    else if (k == "coated_shock_bubble") ic.kind = DIMHIP_IC_COATED_SHOCK_BUBBLE;
+    else throw std::runtime_error("dim::initialise_dim_from_config: unsupported initial_condition: " + k);
+    const auto slot = [&](int id) {
+        for (int s = 0; s < (int)c.materials.size(); ++s)
+            if (c.materials[s].id == id) return s;
+        throw std::runtime_error("dim::material_slot_from_id: material id not found");
+    };
+    for (int d = 0; d < DIM; ++d) {
+        ic.domain_min[d] = c.domain_min[d];
+        ic.domain_max[d] = c.domain_max[d];
+        ic.planar_normal[d] = c.planar_normal[d];
+        ic.bubble_center[d] = c.bubble_center[d];
+    }
+    ic.interface_thickness = c.interface_thickness;
+    ic.sharpness_alpha = c.interface_sharpness_alpha;
+    if (ic.kind == DIMHIP_IC_REGIONS || ic.kind == DIMHIP_IC_PLANAR_REGIONS) {
+        if (c.regions.size() > DIMHIP_MAX_REGIONS) throw std::runtime_error("dimhip_initialise: too many regions");
+        ic.nregions = (int)c.regions.size();
+        for (size_t i = 0; i < c.regions.size(); ++i) {
+            const auto& r = c.regions[i];
+            auto& o = ic.regions[i];
+            for (int d = 0; d < DIM; ++d) {
+                o.lower[d] = r.lower[d];
+                o.upper[d] = r.upper[d];
+                o.vel[d] = r.vel[d];
+            }
+            o.rho = r.rho;
+            o.p = r.p;
+            o.material = slot(r.material_id);
+        }
+    } else {
+        const auto state = [&](const PhaseState<DIM>& ps, dimhip_ic_state& o) {
+            o.rho = ps.rho;
+            o.p = ps.p;
+            for (int d = 0; d < DIM; ++d) o.vel[d] = ps.vel[d];
+            o.material = slot(ps.material);
+        };
+        state(c.left, ic.left);
+        state(c.right, ic.right);
+        state(c.bubble, ic.bubble);
+        if (ic.kind == DIMHIP_IC_COATED_SHOCK_BUBBLE) state(c.film, ic.film);
+        else ic.film.material = -1;
+        ic.shock_axis = c.shock_axis;
+        ic.shock_position = c.shock_position;
+        ic.bubble_radius = c.bubble_radius;
+        ic.film_radius = c.film_radius;
+    }
+    return ic;
+}
+
 class Context {
 public:
     explicit Context(const dimhip_desc& d)
@@ -151,7 +216,10 @@ void run_case(const CaseConfig<DIM>& c, const std::array<int, DIM>& N, const Opt
     Context ctx(desc);
     long long ncells = 1;
     for (int d = 0; d < DIM; ++d) ncells *= N[d];
-    {
+    if (opt.device_ic) {
+        const dimhip_ic ic = make_ic<DIM>(c);
+        check(dimhip_initialise(ctx.get(), &ic), ctx.get());
+    } else {
         const InitialState<DIM> ic(c, N, nmat);
         const std::vector<double> prim = ic.rows();
         check(dimhip_upload_primitive(ctx.get(), prim.data(), (size_t)ncells), ctx.get());
@@ -248,6 +316,7 @@ inline Options parse_args(int argc, char** argv)
     for (int i = 2; i < argc; ++i) {
         const std::string a = argv[i];
         if (a == "--graph") o.graph = true;
+        else if (a == "--device-ic") o.device_ic = true;
         else if (a == "--precision" && i + 1 < argc) {
             const std::string v = argv[++i];
             if (v == "fp64") o.precision = DIMHIP_FP64;

@@ -33,10 +33,11 @@ class Case:
     materials: list
     bc_lo: tuple
     bc_hi: tuple
-    U: np.ndarray  # (ncells, nv) float64
+    U: np.ndarray  # (ncells, nv) float64, or None when only `ic` is given
     cfl: float = 0.8
     lambda_model: str = "kapila"
     alpha_source_floor: float = 0.0
+    ic: dict = None  # keyword arguments of dimhip.make_ic describing the same initial state
 
     @property
     def nmat(self):
@@ -106,15 +107,31 @@ def rankine_hugoniot(gamma, mach, rho1, p1):
 
 def shock_bubble(N, lo, hi, mats, air_slot, bubble_slot, pre, post, bubble, centre, radius,
                  shock_axis, shock_position, thickness, sharpness=2.0, bc_lo=None, bc_hi=None,
-                 name="shock_bubble", bubble_shape=None):
+                 name="shock_bubble", bubble_shape=None, build_state=True):
     """Diffuse shock-bubble state (dim_initial_conditions.hpp:331-404).
 
     pre/post/bubble: (rho, vel tuple, p).  Post-shock gas sits at x >= shock_position.
     bubble_shape: optional callable(coords) -> signed distance (negative inside),
-    replacing the sphere.
+    replacing the sphere (no device description then).  With build_state=False
+    the packed state is not built on the host: the case carries only the
+    description for the device initialiser.
     """
     dim = len(N)
     dx = tuple((hi[d] - lo[d]) / N[d] for d in range(dim))
+    bc_lo = bc_lo or tuple(["transmissive"] * dim)
+    bc_hi = bc_hi or tuple(["transmissive"] * dim)
+    ic = None
+    if bubble_shape is None:
+        ic = dict(kind="shock_bubble", domain_min=tuple(lo), domain_max=tuple(hi), thickness=thickness,
+                  sharpness=sharpness, shock_axis=shock_axis, shock_position=shock_position,
+                  left=(pre[0], tuple(pre[1]), pre[2], air_slot),
+                  right=(post[0], tuple(post[1]), post[2], air_slot),
+                  bubble=(bubble[0], tuple(bubble[1]), bubble[2], bubble_slot),
+                  centre=tuple(centre), radius=radius)
+    if not build_state:
+        if ic is None:
+            raise ValueError("a custom bubble shape needs the host-built state")
+        return Case(name, dim, tuple(N), dx, list(mats), tuple(bc_lo), tuple(bc_hi), None, ic=ic)
     X = _centres(N, lo, dx)
     if bubble_shape is None:
         r2 = sum((X[d] - centre[d]) ** 2 for d in range(dim))
@@ -145,9 +162,7 @@ def shock_bubble(N, lo, hi, mats, air_slot, bubble_slot, pre, post, bubble, cent
     alpha = np.maximum(alpha, 0.0)
     alpha /= alpha.sum(axis=1, keepdims=True)
     U = prim_to_packed(mats, alpha, rho, vel, p)
-    bc_lo = bc_lo or tuple(["transmissive"] * dim)
-    bc_hi = bc_hi or tuple(["transmissive"] * dim)
-    return Case(name, dim, tuple(N), dx, list(mats), tuple(bc_lo), tuple(bc_hi), U)
+    return Case(name, dim, tuple(N), dx, list(mats), tuple(bc_lo), tuple(bc_hi), U, ic=ic)
 
 
 def helium_shock_bubble_2d(nx=2048, ny=1024):
@@ -180,7 +195,7 @@ def stiffened_hugoniot(gamma, pinf, rho1, p1, p2):
     return rho2, u2
 
 
-def bubble_collapse_3d(n=512):
+def bubble_collapse_3d(n=512, build_state=True):
     """Air bubble (ideal, gamma=1.4, rho=1, r=0.15, centre of the unit cube) in
     water (stiffened gas gamma=4.4, p_inf=6e8, rho=1000, p=1e5), struck by a
     planar 1e9 Pa liquid shock entering from x=0; transmissive on every face."""
@@ -193,7 +208,7 @@ def bubble_collapse_3d(n=512):
                         pre=(rho2, (u2, 0.0, 0.0), p2), post=(rho1, (0.0, 0.0, 0.0), p1),
                         bubble=(1.0, (0.0, 0.0, 0.0), p1), centre=(0.5, 0.5, 0.5), radius=0.15,
                         shock_axis=0, shock_position=0.2, thickness=2.0 * h,
-                        name=f"bubble_collapse_3d_{n}")
+                        name=f"bubble_collapse_3d_{n}", build_state=build_state)
     return case
 
 

@@ -148,6 +148,58 @@ int dimhip_boundary_flux_integral(dimhip_ctx* ctx, double* out);
  * reference's initialisers do per cell (dim_initial_conditions.hpp:531-628). */
 int dimhip_upload_primitive(dimhip_ctx* ctx, const double* prim, size_t ncells);
 
+/* Initial conditions generated on the device (initialise_dim_from_config,
+ * dim_initial_conditions.hpp:16-628): region boxes ("regions"), oblique slabs
+ * ("planar_regions"), "shock_bubble" and "coated_shock_bubble", with the
+ * reference's tanh blending and prim_to_cons.  Materials are given as slots
+ * 0..nmat-1.  The arithmetic is the reference's except tanh, the device
+ * library's, which can differ from the host C library's in the last bit:
+ * diffuse cells may then differ from the reference's initial state at the
+ * 1e-16 level (sharp interfaces are identical).  dimhip_upload_primitive
+ * with host-built rows gives the reference's state bit for bit. */
+#define DIMHIP_MAX_REGIONS 16
+enum {
+    DIMHIP_IC_REGIONS = 0,
+    DIMHIP_IC_PLANAR_REGIONS = 1,
+    DIMHIP_IC_SHOCK_BUBBLE = 2,
+    DIMHIP_IC_COATED_SHOCK_BUBBLE = 3
+};
+
+typedef struct {
+    double lower[3];
+    double upper[3];
+    double rho;
+    double vel[3];
+    double p;
+    int material;
+} dimhip_ic_region;
+
+typedef struct {
+    double rho;
+    double vel[3];
+    double p;
+    int material;
+} dimhip_ic_state;
+
+typedef struct {
+    int kind;                        /* DIMHIP_IC_* */
+    double domain_min[3];
+    double domain_max[3];
+    double interface_thickness;
+    double sharpness_alpha;          /* interface_sharpness_alpha */
+    double planar_normal[3];         /* planar_regions: the un-normalised normal */
+    int nregions;
+    dimhip_ic_region regions[DIMHIP_MAX_REGIONS];
+    int shock_axis;
+    double shock_position;
+    dimhip_ic_state left, right, bubble, film;
+    double bubble_center[3];
+    double bubble_radius;
+    double film_radius;
+} dimhip_ic;
+
+int dimhip_initialise(dimhip_ctx* ctx, const dimhip_ic* ic);
+
 /* The values of the solution CSV rows (dim_write_csv.hpp:78-121), computed on
  * the device in FP64 from the current state: per cell
  *   rho, vel[dim], p, e, alpha[nmat], rho_mat[nmat], mass[nmat]
