@@ -33,6 +33,10 @@ WORKLOADS = {
 }
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -44,7 +48,15 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=1000,
                     help="steps of the batched 64-instance companion measurement (0: skip)")
     ap.add_argument("--batch-count", type=int, default=64)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=8,
+                    help="steps of the reference CPU solver timed on the host (about 10-30 s)")
+    ap.add_argument("--accuracy-steps", type=int, default=-1,
+                    help="steps of the FP32-mixed vs FP64 comparison at 4096x2048 (-1: as --fp32-steps, 0: skip)")
+    ap.add_argument("--c5-steps", type=int, default=500,
+                    help="steps of the 3D 512^3 FP32-mixed bubble-collapse measurement (0: skip)")
+    ap.add_argument("--c5-n", type=int, default=512)
+    ap.add_argument("--tube", action=argparse.BooleanOptionalAction, default=True,
+                    help="the 1D water/air tube full run on the GPU and in the reference's CPU solver")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="N",
                     help="only run a device copy of N doubles (calibrates HBM byte counters) and exit")
@@ -109,6 +121,98 @@ def measure(dimhip, workloads, grid, precision, steps, warmup):
     return {"case": case, "value": case.ncells * steps / wall, "wall": wall, "ms": ms, "counts": counts}
 
 
+def field_l1(a, b):
+    """Normalised L1, sum|a-b| / sum|b| (tools/quant/reference_error_metrics.hpp:123-137)."""
+    import numpy as np
+    return float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
+
+
+def accuracy_fp32(dimhip, workloads, grid, steps):
+    """The FP32-mixed form against the FP64 form (bit-identical to the
+    reference's solver) on the same initial state after the same number of
+    steps: normalised L1 of rho, p and alpha_0 from the device's FP64 CSV row
+    values, and each form's relative change of total mass."""
+    case = workloads.helium_shock_bubble_2d(*grid)
+    out = {}
+    rows = {}
+    for prec in ("fp64", "fp32_mixed"):
+        with dimhip.Solver.from_case(case, precision=prec) as s:
+            m0 = s.conservation()[case.nmat]
+            t, n = s.advance(steps, case.cfl, 1.0, 0.0, use_graph=True)
+            m1 = s.conservation()[case.nmat]
+            rows[prec] = s.output_rows()
+            out[prec] = {"time": t, "steps": n, "relative_mass_change": (m1 - m0) / m0}
+    D = case.dim
+    a, b = rows["fp32_mixed"], rows["fp64"]
+    return {
+        "workload": f"helium shock-bubble {grid[0]}x{grid[1]}, {steps} steps of each form from the same state",
+        "l1_rho": field_l1(a[:, 0], b[:, 0]),
+        "l1_p": field_l1(a[:, 1 + D], b[:, 1 + D]),
+        "l1_alpha0": field_l1(a[:, 3 + D], b[:, 3 + D]),
+        "time_fp64": out["fp64"]["time"],
+        "time_fp32_mixed": out["fp32_mixed"]["time"],
+        "relative_mass_change_fp64": out["fp64"]["relative_mass_change"],
+        "relative_mass_change_fp32_mixed": out["fp32_mixed"]["relative_mass_change"],
+        "mass_change_difference": out["fp32_mixed"]["relative_mass_change"] - out["fp64"]["relative_mass_change"],
+        "note": "open boundaries: mass changes by the boundary flux; the forms' difference is the FP32 form's "
+                "extra drift (closed-box drift: tests/test_gpu_fp32_mixed.py::test_closed_box_mass_drift)",
+    }
+
+
+def measure_3d(dimhip, workloads, n, steps, warmup):
+    """3D shock-induced air-bubble collapse in water, n^3 cells, FP32-mixed,
+    initial state generated on the device."""
+    case = workloads.bubble_collapse_3d(n, build_state=False)
+    with dimhip.Solver.from_case(case, precision="fp32_mixed") as s:
+        t, _ = s.advance(warmup, case.cfl, 1.0, 0.0, use_graph=False)
+        s.synchronize()
+        s.set_timing(True)
+        t0 = time.perf_counter()
+        t, nt = s.advance(steps, case.cfl, 1.0, t, use_graph=False)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        ms, counts = s.kernel_times()
+    if nt != steps:
+        raise RuntimeError(f"took {nt} steps, expected {steps}")
+    b = sweep_bytes(case, "fp32_mixed")
+    achieved = b / (ms[0] * 1e-3) / 1e9
+    return {
+        "workload": f"3D bubble collapse {n}^3 (water/air, 1e9 Pa shock), fp32_mixed, split x->y->z, CFL 0.8, "
+                    f"{steps} steps, device-generated initial state",
+        "value": case.ncells * steps / wall,
+        "unit": "cell-updates/s",
+        "ms_per_step": wall / steps * 1e3,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "mean_ms": {"sweep": ms[0], "boundary": ms[1], "cfl": ms[2]}, "bytes_per_launch": b},
+    }
+
+
+def measure_tube(dimhip, workloads, cpu):
+    """The 1D water/air tube (1000 cells) run to t = 2.4e-4 s in the FP64 form
+    with graph-captured steps; the reference's solver takes the same number of
+    steps on one host thread."""
+    case = workloads.water_air_tube_1d(1000)
+    with dimhip.Solver.from_case(case, precision="fp64") as s:
+        s.advance(1, case.cfl, 2.4e-4, 0.0, use_graph=True)  # graph capture outside the timing
+        s.upload(case.U)
+        t0 = time.perf_counter()
+        t, n = s.advance(1 << 20, case.cfl, 2.4e-4, 0.0, use_graph=True)
+        wall = time.perf_counter() - t0
+    out = {"workload": "1D water/air shock tube, 1000 cells, FP64, to t = 2.4e-4 s",
+           "steps": n, "final_time": t, "gpu_seconds": wall, "gpu_value": case.ncells * n / wall}
+    if cpu:
+        from oracle.pyoracle import Checker, make_params, ref_available
+        if ref_available():
+            ref = Checker("ref")
+            p = make_params(1, case.materials)
+            _, _, secs = ref.advance(p, case.N, case.dx, case.bc_lo, case.bc_hi, case.U, case.cfl, 1.0,
+                                     nsteps=n, threads=1)
+            out.update({"cpu_seconds": secs, "cpu_value": case.ncells * n / secs, "cpu_cores": 1,
+                        "cpu_kind": "reference"})
+    return out
+
+
 def measure_batch(dimhip, workloads, count, steps):
     """Seeded uneven shock-interface instances, FP32-mixed, advanced together
     (autotuned tiles and schedule, hipGraph-captured steps)."""
@@ -161,6 +265,7 @@ def main():
         return
 
     grid, precision, _ = WORKLOADS[args.workload]
+    progress(f"{args.workload}: {args.warmup} + {args.steps} steps")
     main_run = measure(dimhip, workloads, grid, precision, args.steps, args.warmup)
     case = main_run["case"]
     cells = case.ncells
@@ -207,6 +312,7 @@ def main():
     }
     if args.fp32_steps > 0 and precision == "fp64":
         g32 = WORKLOADS["helium_fp32"][0]
+        progress(f"fp32_mixed {g32[0]}x{g32[1]}: {args.fp32_steps} steps")
         r32 = measure(dimhip, workloads, g32, "fp32_mixed", args.fp32_steps, args.warmup)
         b32 = sweep_bytes(r32["case"], "fp32_mixed")
         a32 = b32 / (r32["ms"][0] * 1e-3) / 1e9
@@ -222,9 +328,21 @@ def main():
                          "mean_ms": {"sweep": r32["ms"][0], "boundary": r32["ms"][1], "cfl": r32["ms"][2]},
                          "bytes_per_launch": b32},
         }
+        acc_steps = args.fp32_steps if args.accuracy_steps < 0 else args.accuracy_steps
+        if acc_steps > 0:
+            progress(f"fp32_mixed vs fp64 accuracy: {acc_steps} steps of each")
+            out["fp32_mixed"]["accuracy_vs_fp64"] = accuracy_fp32(dimhip, workloads, g32, acc_steps)
     if args.batch_steps > 0 and precision == "fp64":
+        progress(f"batched {args.batch_count} instances: {args.batch_steps} steps")
         out["batched"] = measure_batch(dimhip, workloads, args.batch_count, args.batch_steps)
+    if args.c5_steps > 0 and precision == "fp64":
+        progress(f"3D bubble collapse {args.c5_n}^3: {args.c5_steps} steps")
+        out["bubble_collapse_3d"] = measure_3d(dimhip, workloads, args.c5_n, args.c5_steps, min(args.warmup, 5))
+    if args.tube and precision == "fp64":
+        progress("1D water/air tube full run")
+        out["water_air_tube_1d"] = measure_tube(dimhip, workloads, not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
+        progress(f"reference CPU solver: {args.cpu_steps} steps of helium 2048x1024")
         out["cpu_baseline"] = cpu_baseline(workloads.helium_shock_bubble_2d(2048, 1024), args.cpu_steps)
     print(json.dumps(out))
 

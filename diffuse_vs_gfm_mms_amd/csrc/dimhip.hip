@@ -47,6 +47,8 @@ struct dimhip_ctx {
     int cur = 0;
     StepCtl* ctl = nullptr;
     StepCtl* ctl_host = nullptr;  // pinned
+    StepCtl* probe = nullptr;     // pinned: step control read back while steps are in flight
+    hipEvent_t probe_event = nullptr;
     double* staging = nullptr;
     long long staging_cells = 0;
     double* partial = nullptr;
@@ -592,6 +594,8 @@ int dimhip_create(const dimhip_desc* desc, dimhip_ctx** out)
     HIP_OK(nullptr, hipMemsetAsync(c->q[1], 0, bytes, c->stream));
     HIP_OK(nullptr, hipMalloc(&c->ctl, sizeof(StepCtl)));
     HIP_OK(nullptr, hipHostMalloc(&c->ctl_host, sizeof(StepCtl), hipHostMallocDefault));
+    HIP_OK(nullptr, hipHostMalloc(&c->probe, sizeof(StepCtl), hipHostMallocDefault));
+    HIP_OK(nullptr, hipEventCreateWithFlags(&c->probe_event, hipEventDisableTiming));
     c->partial_blocks = (int)std::min<long long>(1024, std::max<long long>(1, g.ncells / 4096));
     HIP_OK(nullptr, hipMalloc(&c->partial, sizeof(double) * (size_t)c->partial_blocks * 16));
     HIP_OK(nullptr, hipMalloc(&c->totals, sizeof(double) * 16));
@@ -629,6 +633,8 @@ int dimhip_destroy(dimhip_ctx* c)
     hipFree(c->q[1]);
     hipFree(c->ctl);
     hipHostFree(c->ctl_host);
+    hipHostFree(c->probe);
+    if (c->probe_event) hipEventDestroy(c->probe_event);
     hipFree(c->staging);
     hipFree(c->partial);
     hipFree(c->totals);
@@ -741,16 +747,30 @@ int dimhip_advance(dimhip_ctx* c, int nsteps, double cfl, double t_end, int use_
                 c->graph_cur = c->cur;
                 if (!round_trip) return fail(c, "dimhip_advance: graph does not return to its buffer");
             }
-            while (nsteps >= 2) {
-                HIP_OK(c, hipGraphLaunch(c->graph_exec, c->stream));
-                nsteps -= 2;
-            }
         }
+        // Steps go out in chunks; after each chunk the step-control block is
+        // copied back asynchronously, and once the device time has reached
+        // t_end (or an error was flagged) no further chunk is enqueued.  The
+        // check lags one chunk so the queue never drains: at most one chunk of
+        // (inactive, pass-through) steps is issued past t_end.
+        constexpr int kChunk = 256;
+        bool probing = false;
         while (nsteps > 0) {
-            enqueue_step(c, cfl, -1.0);
-            --nsteps;
+            int n = nsteps < kChunk ? nsteps : kChunk;
+            nsteps -= n;
+            if (graphable)
+                for (; n >= 2; n -= 2) HIP_OK(c, hipGraphLaunch(c->graph_exec, c->stream));
+            for (; n > 0; --n) enqueue_step(c, cfl, -1.0);
+            HIP_OK(c, hipGetLastError());
+            if (nsteps == 0) break;
+            if (probing) {
+                HIP_OK(c, hipEventSynchronize(c->probe_event));
+                if (!(c->probe->time < t_end - 1e-14) || c->probe->err) break;
+            }
+            HIP_OK(c, hipMemcpyAsync(c->probe, c->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, c->stream));
+            HIP_OK(c, hipEventRecord(c->probe_event, c->stream));
+            probing = true;
         }
-        HIP_OK(c, hipGetLastError());
     }
     if (fetch_ctl(c)) return 1;
     if (check_device_error(c)) return 1;
