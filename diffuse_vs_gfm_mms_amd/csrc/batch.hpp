@@ -80,28 +80,37 @@ batch_cfl_kernel(const BatchInst<Pol>* __restrict__ insts, const Work* __restric
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
         const Work it = work[w];
         const BatchInst<Pol>& I = insts[it.inst];
-        double cand = bits_to_double(kInfBits);
+        double cand = cfl_identity<Pol>();
         for (long long c = it.a + (long long)threadIdx.x; c < it.b; c += blockDim.x) {
             typename Pol::template Cell<2, 2> U;
             Pol::template load<2, 2>(I.q[cur], I.g.pitch, c, U);
-            cand = smin(cand, Pol::template cfl_candidate<2, 2>(I.P, U, I.cfl));
+            cand = cfl_combine<Pol>(cand, Pol::template cfl_value<2, 2>(I.P, U, I.cfl));
         }
-        block_min_atomic<256>(cand, &I.ctl->dtmin[I.ctl->slot]);
+        cfl_block_atomic<Pol, 256>(cand, I.ctl, I.ctl->slot, w);
     }
 }
 
+// one 64-thread block per instance: reduce the instance's dt slots
 template<typename Pol>
-__global__ void batch_begin_step_kernel(const BatchInst<Pol>* __restrict__ insts, int n, int reset_current)
+__global__ void __launch_bounds__(64)
+batch_begin_step_kernel(const BatchInst<Pol>* __restrict__ insts, int n, int reset_current)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x;
     if (i >= n) return;
     StepCtl* ctl = insts[i].ctl;
+    const int s = ctl->slot;
+    const int k = threadIdx.x;
     if (reset_current) {
-        ctl->dtmin[ctl->slot] = kInfBits;
+        dt_slots(ctl, s)[k * kDtStride] = kInfBits;
+        if (k == 0) ctl->dtmin[s] = kInfBits;
         return;
     }
-    const int s = ctl->slot;
-    const double m = bits_to_double(ctl->dtmin[s]);
+    double m = bits_to_double(dt_slots(ctl, s)[k * kDtStride]);
+    dt_slots(ctl, s ^ 1)[k * kDtStride] = kInfBits;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = smin(m, __shfl_xor(m, off, 64));
+    if (k != 0) return;
+    ctl->dtmin[s] = double_to_bits(m);
     int active = ctl->time < ctl->t_end - 1e-14;
     const double dt_max = ctl->t_end - ctl->time;
     const double dt = smin(dt_max, m);

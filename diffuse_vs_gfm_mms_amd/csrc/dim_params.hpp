@@ -64,4 +64,20 @@ struct StepCtl {
 
 enum : int { ERR_NONFINITE_HLLC = 1, ERR_NONPOSITIVE_DT = 2 };
 
+// The CFL minimum of a launch is collected over kDtSlots addresses one
+// 128-byte line apart (block b writes slot b % kDtSlots), so that thousands
+// of blocks' atomics do not serialise on one address; the step's first
+// kernel reduces the slots into StepCtl::dtmin.  The slots of both parities
+// follow the StepCtl in the same allocation of kCtlAllocBytes.
+constexpr int kDtSlots = 64;
+constexpr int kDtStride = 16;  // unsigned long longs per 128-byte line
+constexpr size_t kCtlBytes = 256;
+constexpr size_t kCtlAllocBytes = kCtlBytes + 2 * kDtSlots * kDtStride * sizeof(unsigned long long);
+static_assert(sizeof(StepCtl) <= kCtlBytes, "StepCtl outgrew its header");
+
+__host__ __device__ inline unsigned long long* dt_slots(StepCtl* ctl, int parity)
+{
+    return (unsigned long long*)((char*)ctl + kCtlBytes) + (size_t)parity * kDtSlots * kDtStride;
+}
+
 }  // namespace dimhip

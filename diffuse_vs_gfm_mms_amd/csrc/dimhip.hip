@@ -160,8 +160,10 @@ CflArgs make_cfl_args(const dimhip_ctx* c, double cfl, bool unsplit)
         a.dx[d] = c->grid.dx[d];
         a.cfl_dx[d] = cfl * c->grid.dx[d];
         a.inv_cfl_dx[d] = 1.0 / a.cfl_dx[d];
+        a.inv_dx[d] = 1.0 / c->grid.dx[d];
     }
     a.cfl = cfl;
+    a.inv_cfl = 1.0 / cfl;
     a.unsplit = unsplit ? 1 : 0;
     return a;
 }
@@ -357,8 +359,11 @@ int validate_desc(const dimhip_desc* d, std::string& msg)
     return 0;
 }
 
+// 2 * kDtSlots threads: the header and every dt slot of both parities
 __global__ void init_ctl_kernel(StepCtl* ctl)
 {
+    if (threadIdx.x < 2 * kDtSlots)
+        dt_slots(ctl, threadIdx.x / kDtSlots)[(threadIdx.x % kDtSlots) * kDtStride] = kInfBits;
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         ctl->dtmin[0] = kInfBits;
         ctl->dtmin[1] = kInfBits;
@@ -422,11 +427,11 @@ void enqueue_step(dimhip_ctx* c, double cfl, double dt_max_host)
     const int D = c->desc.dim;
     if (!c->dt_ready || c->dt_ready_cfl != cfl) {
         const int e = ev_begin(c);
-        reset_dtmin_kernel<<<1, 1, 0, c->stream>>>(c->ctl, 0);
+        reset_dtmin_kernel<<<1, kDtSlots, 0, c->stream>>>(c->ctl, 0);
         c->ops->cfl(c, state_ptr(c, c->cur), unsplit, cfl);
         ev_end(c, e, KC_CFL);
     }
-    begin_step_kernel<<<1, 1, 0, c->stream>>>(c->ctl, dt_max_host);
+    begin_step_kernel<<<1, kDtSlots, 0, c->stream>>>(c->ctl, dt_max_host);
     const bool fuse = fusable(c);
     c->dt_ready_cfl = cfl;
     if (!unsplit) {
@@ -592,7 +597,7 @@ int dimhip_create(const dimhip_desc* desc, dimhip_ctx** out)
     HIP_OK(nullptr, hipMalloc(&c->q[1], bytes));
     HIP_OK(nullptr, hipMemsetAsync(c->q[0], 0, bytes, c->stream));
     HIP_OK(nullptr, hipMemsetAsync(c->q[1], 0, bytes, c->stream));
-    HIP_OK(nullptr, hipMalloc(&c->ctl, sizeof(StepCtl)));
+    HIP_OK(nullptr, hipMalloc(&c->ctl, kCtlAllocBytes));
     HIP_OK(nullptr, hipHostMalloc(&c->ctl_host, sizeof(StepCtl), hipHostMallocDefault));
     HIP_OK(nullptr, hipHostMalloc(&c->probe, sizeof(StepCtl), hipHostMallocDefault));
     HIP_OK(nullptr, hipEventCreateWithFlags(&c->probe_event, hipEventDisableTiming));
@@ -615,7 +620,7 @@ int dimhip_create(const dimhip_desc* desc, dimhip_ctx** out)
     HIP_OK(nullptr, hipMalloc(&c->flux_out, sizeof(double) * 16));
     HIP_OK(nullptr, hipMalloc(&c->flux_integral, sizeof(double) * 16));
     HIP_OK(nullptr, hipMemsetAsync(c->flux_integral, 0, sizeof(double) * 16, c->stream));
-    init_ctl_kernel<<<1, 1, 0, c->stream>>>(c->ctl);
+    init_ctl_kernel<<<1, 2 * kDtSlots, 0, c->stream>>>(c->ctl);
     HIP_OK(nullptr, hipGetLastError());
     HIP_OK(nullptr, hipStreamSynchronize(c->stream));
     *out = c.release();
@@ -785,8 +790,9 @@ int dimhip_compute_dt(dimhip_ctx* c, double cfl, double dt_max, double* dt_out)
     if (cfl <= 0.0) return fail(c, "dim::compute_dt_cfl: cfl must be positive");
     if (dt_max <= 0.0) return fail(c, "dim::compute_dt_cfl: dt_max must be positive");
     HIP_OK(c, hipSetDevice(c->desc.device));
-    reset_dtmin_kernel<<<1, 1, 0, c->stream>>>(c->ctl, 0);
+    reset_dtmin_kernel<<<1, kDtSlots, 0, c->stream>>>(c->ctl, 0);
     c->ops->cfl(c, state_ptr(c, c->cur), c->desc.time_update == DIMHIP_UNSPLIT, cfl);
+    reduce_dtmin_kernel<<<1, kDtSlots, 0, c->stream>>>(c->ctl);
     HIP_OK(c, hipGetLastError());
     if (fetch_ctl(c)) return 1;
     const double m = __builtin_bit_cast(double, c->ctl_host->dtmin[c->ctl_host->slot]);
@@ -1289,15 +1295,14 @@ void batch_sweep_y(dimhip_batch* b, bool fuse)
 void batch_enqueue_step(dimhip_batch* b, double cfl)
 {
     const int n = (int)b->ctx.size();
-    const int tb = (n + 63) / 64;
     if (!b->dt_ready || b->dt_ready_cfl != cfl) {
         const int e = bev_begin(b);
-        batch_begin_step_kernel<BPol><<<tb, 64, 0, b->stream>>>(b->d_insts, n, 1);
+        batch_begin_step_kernel<BPol><<<n, kDtSlots, 0, b->stream>>>(b->d_insts, n, 1);
         batch_cfl_kernel<BPol><<<std::max(1, std::min(b->nwork[3], 4096)), 256, 0, b->stream>>>(
             b->d_insts, b->d_work[3], b->nwork[3], b->cur);
         bev_end(b, e, KC_CFL);
     }
-    batch_begin_step_kernel<BPol><<<tb, 64, 0, b->stream>>>(b->d_insts, n, 0);
+    batch_begin_step_kernel<BPol><<<n, kDtSlots, 0, b->stream>>>(b->d_insts, n, 0);
     const int bcb = std::max(1, std::min(b->nwork[2], 2048));
     int e = bev_begin(b);
     batch_sweep_x(b);
@@ -1498,12 +1503,14 @@ int dimhip_batch_autotune(dimhip_batch* b, double cfl, int trial_steps, char* re
     // back up every instance's state and control block
     std::vector<void*> backup(n, nullptr);
     std::vector<StepCtl> ctl_backup(n);
+    std::vector<std::vector<char>> slot_backup(n, std::vector<char>(kCtlAllocBytes));
     for (int i = 0; i < n; ++i) {
         dimhip_ctx* c = b->ctx[i];
         const size_t bytes = (size_t)c->grid.pitch * c->cell_bytes;
         BHIP_OK(b, hipMalloc(&backup[i], bytes));
         BHIP_OK(b, hipMemcpy(backup[i], c->q[b->cur], bytes, hipMemcpyDeviceToDevice));
-        BHIP_OK(b, hipMemcpy(&ctl_backup[i], c->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost));
+        BHIP_OK(b, hipMemcpy(slot_backup[i].data(), c->ctl, kCtlAllocBytes, hipMemcpyDeviceToHost));
+        std::memcpy(&ctl_backup[i], slot_backup[i].data(), sizeof(StepCtl));
     }
     const int cur0 = b->cur;
     const bool ready0 = b->dt_ready;
@@ -1550,7 +1557,7 @@ int dimhip_batch_autotune(dimhip_batch* b, double cfl, int trial_steps, char* re
                         dimhip_ctx* c = b->ctx[i];
                         BHIP_OK(b, hipMemcpy(c->q[cur0], backup[i], (size_t)c->grid.pitch * c->cell_bytes,
                                              hipMemcpyDeviceToDevice));
-                        BHIP_OK(b, hipMemcpy(c->ctl, &ctl_backup[i], sizeof(StepCtl), hipMemcpyHostToDevice));
+                        BHIP_OK(b, hipMemcpy(c->ctl, slot_backup[i].data(), kCtlAllocBytes, hipMemcpyHostToDevice));
                         c->cur = cur0;
                     }
                     b->dt_ready = ready0;

@@ -14,13 +14,20 @@ struct Layout {
     static constexpr int NF = M + D + 1;
 };
 
-// Start of a step: dt = min(dt_max, CFL minimum) exactly as the reference's
-// running std::min from dt = dt_max (cfl.hpp:43-63); clear the next slot.
-__global__ void begin_step_kernel(StepCtl* ctl, double dt_max_host)
+// Start of a step (64 threads): the CFL minimum is the min over the current
+// parity's dt slots (min is exact in any order); dt = min(dt_max, minimum)
+// exactly as the reference's running std::min from dt = dt_max
+// (cfl.hpp:43-63); the next parity's slots are cleared.
+__global__ void __launch_bounds__(64) begin_step_kernel(StepCtl* ctl, double dt_max_host)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int s = ctl->slot;
-    const double m = bits_to_double(ctl->dtmin[s]);
+    const int k = threadIdx.x;
+    double m = bits_to_double(dt_slots(ctl, s)[k * kDtStride]);
+    dt_slots(ctl, s ^ 1)[k * kDtStride] = kInfBits;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = smin(m, __shfl_xor(m, off, 64));
+    if (k != 0) return;
+    ctl->dtmin[s] = double_to_bits(m);
     double dt_max = dt_max_host;
     int active = 1;
     if (ctl->use_t_end) {
@@ -38,9 +45,22 @@ __global__ void begin_step_kernel(StepCtl* ctl, double dt_max_host)
     ctl->dtmin[s ^ 1] = kInfBits;
 }
 
-__global__ void reset_dtmin_kernel(StepCtl* ctl, int which_rel)
+// 64 threads: clear the dt slots of parity slot ^ which_rel
+__global__ void __launch_bounds__(64) reset_dtmin_kernel(StepCtl* ctl, int which_rel)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) ctl->dtmin[ctl->slot ^ which_rel] = kInfBits;
+    const int s = ctl->slot ^ which_rel;
+    dt_slots(ctl, s)[threadIdx.x * kDtStride] = kInfBits;
+    if (threadIdx.x == 0) ctl->dtmin[s] = kInfBits;
+}
+
+// 64 threads: StepCtl::dtmin[slot] = min over the current parity's dt slots
+__global__ void __launch_bounds__(64) reduce_dtmin_kernel(StepCtl* ctl)
+{
+    const int s = ctl->slot;
+    double m = bits_to_double(dt_slots(ctl, s)[threadIdx.x * kDtStride]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = smin(m, __shfl_xor(m, off, 64));
+    if (threadIdx.x == 0) ctl->dtmin[s] = double_to_bits(m);
 }
 
 __global__ void set_active_kernel(StepCtl* ctl, double dt)

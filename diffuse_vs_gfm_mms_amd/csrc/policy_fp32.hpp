@@ -737,7 +737,11 @@ __device__ __forceinline__ bool hllc(const EosMixed& P, const St<D, M>& UL, cons
 }  // namespace f32
 
 struct Fp32Mixed {
-    static constexpr int kMinWavesPerEU = 1;
+    static constexpr int kMinWavesPerEU = 8;
+    // CFL reduction: each cell contributes its wave rate max_d (|u_d|+c)/(cfl dx_d),
+    // combined by max in FP64; the block's dt is 1/rate (reciprocal is monotone,
+    // so this equals the minimum of the per-cell 1/rate)
+    static constexpr bool kCflRate = true;
     using R = float;
     using Eos = EosMixed;
     template<int D, int M> using St = f32::St<D, M>;
@@ -1027,11 +1031,11 @@ struct Fp32Mixed {
     template<int D, int M>
     static __device__ __forceinline__ void repair_cell(const Eos& P, Cell<D, M>& U) { f32::repair<D, M>(P, U); }
 
-    // FP64 candidate min_d cfl*dx_d/(|u_d|+c) (cfl.hpp:43-63) from the FP32
-    // wave speed, as 1 / max_d((|u_d|+c) / (cfl*dx_d)): one FP64 division
-    // per cell; equal to the reference's form up to FP64 rounding
+    // FP64 wave rate of the cell, 1/dt of cfl.hpp:43-63 (split: max_d
+    // (|u_d|+c)/(cfl dx_d); unsplit: sum_d (|u_d|+c)/dx_d / cfl), from the
+    // FP32 wave speed; 0 when no direction has |u_d|+c > 1e-14
     template<int D, int M>
-    static __device__ __forceinline__ double cfl_candidate(const Eos& P, const Cell<D, M>& C, const CflArgs& a)
+    static __device__ __forceinline__ double cfl_value(const Eos& P, const Cell<D, M>& C, const CflArgs& a)
     {
         St<D, M> U;
         to_state<D, M>(C, U);
@@ -1048,12 +1052,12 @@ struct Fp32Mixed {
                 const double ws = (double)__builtin_fabsf(Q.vel[d]) + c;
                 if (ws > 1e-14) rate = __builtin_fmax(rate, ws * a.inv_cfl_dx[d]);
             }
-            return rate > 0.0 ? 1.0 / rate : bits_to_double(kInfBits);
+            return rate;
         }
         double spectral = -0.0;
 #pragma unroll
-        for (int d = 0; d < D; ++d) spectral += ((double)__builtin_fabsf(Q.vel[d]) + c) / a.dx[d];
-        return spectral > 1e-14 ? a.cfl / spectral : bits_to_double(kInfBits);
+        for (int d = 0; d < D; ++d) spectral += ((double)__builtin_fabsf(Q.vel[d]) + c) * a.inv_dx[d];
+        return spectral > 1e-14 ? spectral * a.inv_cfl : 0.0;
     }
 
     template<int D, int M>

@@ -11,7 +11,9 @@ struct CflArgs {
     double cfl_dx[3];
     double inv_cfl_dx[3];  // 1 / (cfl * dx): the FP32-mixed form's wave-rate reduction
     double dx[3];
+    double inv_dx[3];
     double cfl;
+    double inv_cfl;
     int unsplit;
 };
 
@@ -21,6 +23,8 @@ __device__ __forceinline__ unsigned long long double_to_bits(double x) { return 
 
 struct Fp64 {
     static constexpr int kMinWavesPerEU = 4;
+    // CFL reduction: each cell contributes its dt candidate, combined by min
+    static constexpr bool kCflRate = false;
     using R = double;
     using Eos = EosParams;
     template<int D, int M> using St = State<D, M>;
@@ -207,7 +211,7 @@ struct Fp64 {
 
     // CFL candidate (cfl.hpp:46-63 / 94-116)
     template<int D, int M>
-    static __device__ __forceinline__ double cfl_candidate(const Eos& P, const Cell<D, M>& U, const CflArgs& a)
+    static __device__ __forceinline__ double cfl_value(const Eos& P, const Cell<D, M>& U, const CflArgs& a)
     {
         Prim<D, M> Q;
         dimhip::cons_to_prim<D, M>(P, U, Q);

@@ -64,24 +64,81 @@ __device__ __forceinline__ void get_state(const R* sh, int S, int slot, int off,
 
 // Block-wide min of positive doubles (or +inf) and one atomic per block on
 // the bit pattern (positive doubles order like unsigned integers).
-template<int NT>
-__device__ __forceinline__ void block_min_atomic(double v, unsigned long long* target)
+// CFL reduction of one block into *target (dt slot, bit-ordered atomicMin of
+// a positive double).  A policy contributes either dt candidates combined by
+// min (the FP64 form: min_cell min_d cfl dx_d/(|u_d|+c), cfl.hpp:43-63) or
+// wave rates combined by max, inverted once per block (the FP32-mixed form).
+template<typename Pol>
+__device__ __forceinline__ double cfl_identity()
 {
-    __shared__ double red[NT / 64 > 0 ? NT / 64 : 1];
+    return Pol::kCflRate ? 0.0 : bits_to_double(kInfBits);
+}
+
+template<typename Pol>
+__device__ __forceinline__ double cfl_combine(double a, double b)
+{
+    return Pol::kCflRate ? smax(a, b) : smin(a, b);
+}
+
+// Non-negative doubles (and +inf) order like their bit patterns as unsigned
+// integers, so the wave reduction runs on bits: min (candidates) or max
+// (rates); a NaN pattern is above +inf and never wins a min, as in the
+// reference's running std::min.
+template<bool MAX>
+__device__ __forceinline__ unsigned long long u64_combine(unsigned long long a, unsigned long long b)
+{
+    return MAX ? (a > b ? a : b) : (a < b ? a : b);
+}
+
+// v from lane (per DPP control) of this wave; lanes outside row_mask get `idle`
+template<int CTRL, int ROW_MASK>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v, unsigned long long idle)
+{
+    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)idle, (int)(unsigned)v, CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(idle >> 32), (int)(unsigned)(v >> 32), CTRL, ROW_MASK,
+                                               0xF, false);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+
+// Reduction over a full wave with DPP moves (quad swaps, half-row and row
+// mirrors, row broadcasts 15 and 31): lane 63 ends with the wave's result.
+template<bool MAX>
+__device__ __forceinline__ unsigned long long wave_reduce_u64(unsigned long long v)
+{
+    constexpr unsigned long long idle = MAX ? 0ull : ~0ull;
+    v = u64_combine<MAX>(v, dpp_u64<0xB1, 0xF>(v, idle));   // quad_perm [1,0,3,2]
+    v = u64_combine<MAX>(v, dpp_u64<0x4E, 0xF>(v, idle));   // quad_perm [2,3,0,1]
+    v = u64_combine<MAX>(v, dpp_u64<0x141, 0xF>(v, idle));  // row_half_mirror
+    v = u64_combine<MAX>(v, dpp_u64<0x140, 0xF>(v, idle));  // row_mirror
+    v = u64_combine<MAX>(v, dpp_u64<0x142, 0xA>(v, idle));  // row_bcast:15 into rows 1, 3
+    v = u64_combine<MAX>(v, dpp_u64<0x143, 0xC>(v, idle));  // row_bcast:31 into rows 2, 3
+    const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)v, 63);
+    const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// CFL reduction of one block into its dt slot (bit-ordered atomicMin of a
+// positive double).  A policy contributes either dt candidates combined by
+// min (the FP64 form: min_cell min_d cfl dx_d/(|u_d|+c), cfl.hpp:43-63) or
+// wave rates combined by max, inverted once per block (the FP32-mixed form).
+// Every thread of the block calls it.
+template<typename Pol, int NT>
+__device__ __forceinline__ void cfl_block_atomic(double v, StepCtl* ctl, int parity, int blk)
+{
+    unsigned long long* target = dt_slots(ctl, parity) + (blk % kDtSlots) * kDtStride;
+    __shared__ unsigned long long red[NT / 64 > 0 ? NT / 64 : 1];
     __syncthreads();  // the block may reuse red[] for its next tile
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const double o = __shfl_xor(v, off, 64);
-        v = smin(v, o);
-    }
+    const unsigned long long w = wave_reduce_u64<Pol::kCflRate>(double_to_bits(v));
     const int tid = threadIdx.x + threadIdx.y * blockDim.x + threadIdx.z * blockDim.x * blockDim.y;
-    if ((tid & 63) == 0) red[tid >> 6] = v;
+    if ((tid & 63) == 0) red[tid >> 6] = w;
     __syncthreads();
     if (tid == 0) {
-        double m = red[0];
+        unsigned long long m = red[0];
 #pragma unroll
-        for (int w = 1; w < NT / 64; ++w) m = smin(m, red[w]);
-        if (m < bits_to_double(kInfBits)) atomicMin(target, double_to_bits(m));
+        for (int k = 1; k < NT / 64; ++k) m = u64_combine<Pol::kCflRate>(m, red[k]);
+        const double r = bits_to_double(m);
+        const double dt = Pol::kCflRate ? (r > 0.0 ? 1.0 / r : bits_to_double(kInfBits)) : r;
+        if (dt < bits_to_double(kInfBits)) atomicMin(target, double_to_bits(dt));
     }
 }
 
@@ -293,7 +350,7 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     __syncthreads();
 
     // phase 5: update cell i
-    double cand = bits_to_double(kInfBits);
+    double cand = cfl_identity<Pol>();
     if (own) {
         Cell r = Uc;
         if (n > 1) {
@@ -342,10 +399,12 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
 #pragma unroll
             for (int d = 0; d < D; ++d)
                 if (g.N[d] > 1 && (idx[d] == 0 || idx[d] == g.N[d] - 1)) interior = false;
-            if (interior) cand = Pol::template cfl_candidate<D, M>(P, r, cflargs);
+            if (interior) cand = Pol::template cfl_value<D, M>(P, r, cflargs);
         }
     }
-    if (FUSE_CFL) block_min_atomic<S>(cand, &ctl->dtmin[ctl->slot ^ 1]);
+    if (FUSE_CFL)
+        cfl_block_atomic<Pol, S>(cand, ctl, ctl->slot ^ 1,
+                                 (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)));
 }
 
 // one tile per block, blocks in XCD-aware order
@@ -524,14 +583,14 @@ __global__ void __launch_bounds__(256)
 cfl_kernel(const void* __restrict__ q, Grid g, typename Pol::Eos P, StepCtl* __restrict__ ctl, CflArgs a)
 {
     using Cell = typename Pol::template Cell<D, M>;
-    double cand = bits_to_double(kInfBits);
+    double cand = cfl_identity<Pol>();
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < g.ncells;
          c += (long long)gridDim.x * blockDim.x) {
         Cell U;
         Pol::template load<D, M>(q, g.pitch, c, U);
-        cand = smin(cand, Pol::template cfl_candidate<D, M>(P, U, a));
+        cand = cfl_combine<Pol>(cand, Pol::template cfl_value<D, M>(P, U, a));
     }
-    block_min_atomic<256>(cand, &ctl->dtmin[ctl->slot]);
+    cfl_block_atomic<Pol, 256>(cand, ctl, ctl->slot, (int)blockIdx.x);
 }
 
 // packed per-cell doubles <-> policy storage
