@@ -300,10 +300,15 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     const bool ph3 = in && pos >= 2 && pos < L - 2;
     if (ph3) {
         if (end_cell) {
-            Ulh = U;
-            Urh = U;
-            PUl = Pn;
-            PUr = Pn;
+            // an end cell's faces are the cell itself (UL = UR = U^n and its
+            // primitive): rebuilt from memory rather than held in registers
+            // since phase 1
+            Cell Ue;
+            Pol::template load<D, M>(qin, g.pitch, cell, Ue);
+            Pol::template to_state<D, M>(Ue, Ulh);
+            Urh = Ulh;
+            Pol::template cons_to_prim<D, M>(P, Ulh, PUl);
+            PUr = PUl;
         } else {
             Pr PL, PR, lf, rf;
             get_prim<R, D, M>(sh, S, slot_l, 0, PL);
@@ -352,7 +357,11 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     // phase 5: update cell i
     double cand = cfl_identity<Pol>();
     if (own) {
-        Cell r = Uc;
+        // the cell's input state again (an L2 hit) rather than held in
+        // registers across the four phases
+        Cell Uin;
+        Pol::template load<D, M>(qin, g.pitch, cell, Uin);
+        Cell r = Uin;
         if (n > 1) {
             Fx Fl, Fr;
             R fvl, fvr, afl[M], afr[M];
@@ -385,12 +394,12 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
 #pragma unroll
                 for (int k = 0; k < M; ++k) afr[k] = Pn.alpha[k] * fvr;
             }
-            Pol::template update<D, M>(P, Uc, Pn, rhs_n, Fl, Fr, fvl, fvr, afl, afr, dt, dxs, r);
+            Pol::template update<D, M>(P, Uin, Pn, rhs_n, Fl, Fr, fvl, fvr, afl, afr, dt, dxs, r);
         }
         if (MODE == 1) {
             Cell acc;
             Pol::template load<D, M>(qout, g.pitch, cell, acc);
-            Pol::template accumulate<D, M>(P, acc, r, Uc);
+            Pol::template accumulate<D, M>(P, acc, r, Uin);
             r = acc;
         }
         Pol::template store<D, M>(qout, g.pitch, cell, r);
