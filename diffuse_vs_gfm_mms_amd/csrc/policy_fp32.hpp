@@ -738,6 +738,7 @@ __device__ __forceinline__ bool hllc(const EosMixed& P, const St<D, M>& UL, cons
 
 struct Fp32Mixed {
     static constexpr int kMinWavesPerEU = 8;
+    static constexpr bool kReloadInput = false;
     // CFL reduction: each cell contributes its wave rate max_d (|u_d|+c)/(cfl dx_d),
     // combined by max in FP64; the block's dt is 1/rate (reciprocal is monotone,
     // so this equals the minimum of the per-cell 1/rate)

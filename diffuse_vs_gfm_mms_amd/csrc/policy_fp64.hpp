@@ -23,6 +23,9 @@ __device__ __forceinline__ unsigned long long double_to_bits(double x) { return 
 
 struct Fp64 {
     static constexpr int kMinWavesPerEU = 4;
+    // the sweep re-reads a cell's input state instead of keeping it in
+    // registers across its phases (register pressure, not bandwidth, binds here)
+    static constexpr bool kReloadInput = true;
     // CFL reduction: each cell contributes its dt candidate, combined by min
     static constexpr bool kCflRate = false;
     using R = double;

@@ -301,13 +301,18 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     if (ph3) {
         if (end_cell) {
             // an end cell's faces are the cell itself (UL = UR = U^n and its
-            // primitive): rebuilt from memory rather than held in registers
-            // since phase 1
-            Cell Ue;
-            Pol::template load<D, M>(qin, g.pitch, cell, Ue);
-            Pol::template to_state<D, M>(Ue, Ulh);
+            // primitive); with Pol::kReloadInput rebuilt from memory rather
+            // than held in registers since phase 1
+            if constexpr (Pol::kReloadInput) {
+                Cell Ue;
+                Pol::template load<D, M>(qin, g.pitch, cell, Ue);
+                Pol::template to_state<D, M>(Ue, Ulh);
+                Pol::template cons_to_prim<D, M>(P, Ulh, PUl);
+            } else {
+                Ulh = U;
+                PUl = Pn;
+            }
             Urh = Ulh;
-            Pol::template cons_to_prim<D, M>(P, Ulh, PUl);
             PUr = PUl;
         } else {
             Pr PL, PR, lf, rf;
@@ -357,10 +362,11 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     // phase 5: update cell i
     double cand = cfl_identity<Pol>();
     if (own) {
-        // the cell's input state again (an L2 hit) rather than held in
-        // registers across the four phases
+        // with Pol::kReloadInput the cell's input state is read again (an L2
+        // hit) rather than held in registers across the four phases
         Cell Uin;
-        Pol::template load<D, M>(qin, g.pitch, cell, Uin);
+        if constexpr (Pol::kReloadInput) Pol::template load<D, M>(qin, g.pitch, cell, Uin);
+        else Uin = Uc;
         Cell r = Uin;
         if (n > 1) {
             Fx Fl, Fr;
