@@ -10,9 +10,11 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "dimhip.hip")
+# the C ABI and one translation unit per arithmetic policy, compiled in parallel
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("dimhip.hip", "ops_fp64.hip", "ops_fp64na.hip", "ops_fp32.hip")]
 OUT_DIR = os.path.join(HERE, "_lib")
 OUT = os.path.join(OUT_DIR, "libdimhip.so")
 
@@ -21,7 +23,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # the reference's arithmetic is reproduced operation by operation: no FMA contraction
     "-ffp-contract=off",
     "-fno-fast-math",
@@ -41,15 +42,29 @@ def hipcc():
 
 def build_library(verbose=False):
     os.makedirs(OUT_DIR, exist_ok=True)
-    tmp = OUT + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, SRC]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libdimhip.so")
-    if verbose:
-        noise = "is not a recognized feature for this target"
-        sys.stderr.write("".join(l + "\n" for l in res.stderr.splitlines() if noise not in l))
+    noise = "is not a recognized feature for this target"
+    with tempfile.TemporaryDirectory(prefix="dimhip_build_") as tmpdir:
+        procs = []
+        for src in SOURCES:
+            obj = os.path.join(tmpdir, os.path.basename(src) + ".o")
+            cmd = [hipcc(), *HIPCC_FLAGS, "-c", "-o", obj, src]
+            procs.append((src, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        failed = False
+        for src, _, proc in procs:
+            out, _ = proc.communicate()
+            if proc.returncode != 0:
+                sys.stderr.write(out)
+                failed = True
+            elif verbose:
+                sys.stderr.write("".join(l + "\n" for l in out.splitlines() if noise not in l))
+        if failed:
+            raise RuntimeError("hipcc failed building libdimhip.so")
+        tmp = OUT + ".tmp"
+        cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *[o for _, o, _ in procs]]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError("hipcc failed linking libdimhip.so")
     os.replace(tmp, OUT)
     return OUT
 

@@ -1,0 +1,152 @@
+// context.hpp — the library context (one grid, one stream, double-buffered
+// SoA state, device step control) and the launch interface shared by the
+// library's translation units.  The kernels of each arithmetic policy are
+// instantiated in a translation unit of their own (ops_fp64.hip,
+// ops_fp64na.hip, ops_fp32.hip) through the launcher templates declared here
+// and defined in ops_impl.hpp; dimhip.hip holds the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/dimhip.h"
+#include "ic_kernels.hpp"
+#include "policy_fp32.hpp"
+#include "policy_fp64.hpp"
+#include "sweep.hpp"
+
+namespace dimhip {
+
+constexpr int kSweepL0 = 128, kSweepW0 = 2;   // x sweep: 128 positions x 2 lines
+constexpr int kSweepL1 = 64, kSweepW1 = 8;    // y/z sweep: 8 lines (x) x 64 positions
+
+enum KernelClass { KC_SWEEP = 0, KC_BC = 1, KC_CFL = 2, KC_COUNT = 3 };
+
+struct Ops;
+
+}  // namespace dimhip
+
+struct dimhip_ctx {
+    dimhip_desc desc{};
+    dimhip::Grid grid{};
+    dimhip::EosParams eos{};
+    dimhip::EosParamsNA eos_na{};  // the same constants, for kernels compiled with the noble-Abel law
+    dimhip::EosMixed eosm{};
+    dimhip::BcArgs bc{};
+    int nv = 0;
+    size_t cell_bytes = 0;  // storage bytes per cell
+    hipStream_t stream = nullptr;
+    void* q[2] = {nullptr, nullptr};
+    int cur = 0;
+    dimhip::StepCtl* ctl = nullptr;
+    dimhip::StepCtl* ctl_host = nullptr;  // pinned
+    dimhip::StepCtl* probe = nullptr;     // pinned: step control read back while steps are in flight
+    hipEvent_t probe_event = nullptr;
+    double* staging = nullptr;
+    long long staging_cells = 0;
+    double* partial = nullptr;
+    double* totals = nullptr;
+    int partial_blocks = 0;
+    // driver I/O: primitive rows in, CSV row values out (grown on demand)
+    double* io_buf = nullptr;
+    size_t io_doubles = 0;
+    // boundary flux: per-block partials, last result, running integral
+    long long flux_lines[4] = {0, 0, 0, 0};
+    int flux_blocks = 0;
+    double* flux_partial = nullptr;
+    double* flux_out = nullptr;
+    double* flux_integral = nullptr;
+    bool track_flux = false;
+    bool dt_ready = false;       // ctl->dtmin[slot] holds the CFL minimum of the current state
+    double dt_ready_cfl = 0.0;   // the cfl it was computed with
+    std::string err;
+    const dimhip::Ops* ops = nullptr;
+    // graph for dimhip_advance
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    double graph_cfl = -1.0;
+    int graph_cur = -1;
+    // event timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_pending;  // (class, index of start event)
+    size_t ev_next = 0;
+    double kt_sum[dimhip::KC_COUNT] = {0, 0, 0};
+    long long kt_cnt[dimhip::KC_COUNT] = {0, 0, 0};
+};
+
+namespace dimhip {
+
+// Precision- and shape-specific work of a context, one implementation per
+// (dim, nmat) (dimhip.hip), dispatching on the context's precision form.
+struct Ops {
+    virtual ~Ops() = default;
+    virtual void sweep(dimhip_ctx* c, int dir, int mode, bool fuse, const void* in, void* out) const = 0;
+    virtual void bc(dimhip_ctx* c, void* q, int end_step) const = 0;
+    virtual void cfl(dimhip_ctx* c, const void* q, bool unsplit, double cfl) const = 0;
+    virtual void totals(dimhip_ctx* c, const void* q) const = 0;
+    virtual void pack(dimhip_ctx* c, bool in, double* staging, void* q, long long first, long long count) const = 0;
+    virtual int kat(const EosParams& P, int which, int dir, int count, const double* in0, const double* in1,
+                    double* out0, double* out1, int* err, hipStream_t s) const = 0;
+    virtual void prim_to_packed(dimhip_ctx* c, const double* prim, double* packed, long long count) const = 0;
+    virtual void output_rows(dimhip_ctx* c, const double* packed, double* rows, long long count) const = 0;
+    virtual void boundary_flux(dimhip_ctx* c, const void* q, double* integral) const = 0;
+    virtual void ic_prims(dimhip_ctx* c, const IcDesc& ic, long long first, long long count, double* prim,
+                          int* err) const = 0;
+};
+
+inline CflArgs make_cfl_args(const dimhip_ctx* c, double cfl, bool unsplit)
+{
+    CflArgs a{};
+    for (int d = 0; d < 3; ++d) {
+        a.dx[d] = c->grid.dx[d];
+        a.cfl_dx[d] = cfl * c->grid.dx[d];
+        a.inv_cfl_dx[d] = 1.0 / a.cfl_dx[d];
+        a.inv_dx[d] = 1.0 / c->grid.dx[d];
+    }
+    a.cfl = cfl;
+    a.inv_cfl = 1.0 / cfl;
+    a.unsplit = unsplit ? 1 : 0;
+    return a;
+}
+
+inline int grid_blocks(long long work, int per)
+{
+    long long b = (work + per - 1) / per;
+    if (b < 1) b = 1;
+    if (b > 8192) b = 8192;
+    return (int)b;
+}
+
+
+// Launchers of one arithmetic policy's kernels for one (dim, nmat):
+// explicitly instantiated in that policy's translation unit.
+template<typename Pol, int D, int M>
+struct PolOps {
+    using Eos = typename Pol::Eos;
+    static void sweep(dimhip_ctx* c, int dir, int mode, bool fuse, const void* in, void* out, const Eos& P);
+    static void bc(dimhip_ctx* c, void* q, int end_step, const Eos& P);
+    static void cfl(dimhip_ctx* c, const void* q, bool unsplit, double cfl, const Eos& P);
+    static void totals(dimhip_ctx* c, const void* q, const Eos& P);
+    static void pack(dimhip_ctx* c, bool in, double* staging, void* q, long long first, long long count, const Eos& P);
+    static void boundary_flux(dimhip_ctx* c, const void* q, double* integral, const Eos& PE);
+};
+
+// The FP64 device functions one by one (dimhip_kat), with either parameter block.
+template<int D, int M, typename EP>
+struct KatOps {
+    static int kat(const EP& P, int which, int dir, int count, const double* in0, const double* in1, double* out0,
+                   double* out1, int* err, hipStream_t s);
+};
+
+// Driver input and output rows, FP64 with every supported material law.
+template<int D, int M>
+struct IoOps {
+    static void prim_to_packed(dimhip_ctx* c, const double* prim, double* packed, long long count);
+    static void output_rows(dimhip_ctx* c, const double* packed, double* rows, long long count);
+};
+
+}  // namespace dimhip

@@ -63,44 +63,67 @@ struct Flux {
 };
 
 // ---- per-material EoS (src/euler/eos.hpp) --------------------------------
-// pressure<1> of a zero-momentum Conserved<1> (eos.hpp:451-466, 555-571)
-__device__ __forceinline__ double mat_pressure_c1(const EosParams& P, int k, double rho_in, double E)
+// pressure<1> of a zero-momentum Conserved<1>: ideal and stiffened gas
+// (eos.hpp:451-466, 555-571) and noble-Abel (eos.hpp:665-677, whose
+// internal energy is internal_energy_from_conserved, eos.hpp:13-25)
+template<typename EP>
+__device__ __forceinline__ double mat_pressure_c1(const EP& P, int k, double rho_in, double E)
 {
     const double rho = clamp_min(rho_in);
     const double kinetic = 0.5 * safe_div(0.0, rho);
+    if (EP::kNA && P.kind[k] == EOS_NOBLE_ABEL) {
+        const double e = smax(safe_div(E - kinetic, rho), 0.0);
+        const double denom = safe_denom(1.0 - P.covol[k] * rho, 1e-14);
+        return clamp_min(P.gm1[k] * rho * e / denom);
+    }
     if (P.kind[k] == EOS_STIFFENED)
         return clamp_min(P.gm1[k] * (E - kinetic) - P.gpinf[k]);
     return clamp_min(P.gm1[k] * (E - kinetic));
 }
 
-// internal_energy (eos.hpp:482-489, 587-595); gm1 = gamma-1, gpinf = gamma*p_inf
-__device__ __forceinline__ double mat_internal_energy(const EosParams& P, int k, double rho, double p)
+// internal_energy (eos.hpp:482-489, 587-595, 692-702); gm1 = gamma-1,
+// gpinf = gamma*p_inf.  The noble-Abel form clamps its inputs and divides
+// without a guard, as the reference does.
+template<typename EP>
+__device__ __forceinline__ double mat_internal_energy(const EP& P, int k, double rho, double p)
 {
+    if (EP::kNA && P.kind[k] == EOS_NOBLE_ABEL) {
+        const double r = clamp_min(rho);
+        const double pp = clamp_min(p);
+        return pp * (1.0 - P.covol[k] * r) / (P.gm1[k] * r);
+    }
     if (P.kind[k] == EOS_STIFFENED) return safe_div(p + P.gpinf[k], P.gm1[k] * rho);
     return safe_div(p, P.gm1[k] * rho);
 }
 
 // eos.hpp:16-27
-__device__ __forceinline__ double material_pressure_from_density_energy(const EosParams& P, int k,
+template<typename EP>
+__device__ __forceinline__ double material_pressure_from_density_energy(const EP& P, int k,
                                                                          double rho, double e)
 {
     const double r = clamp_min(rho);
     return mat_pressure_c1(P, k, r, r * smax(e, 0.0));
 }
 
-// eos.hpp:29-40 with sound_speed<1> (eos.hpp:469-479, 574-583)
-__device__ __forceinline__ double material_sound_speed(const EosParams& P, int k, double rho, double p)
+// eos.hpp:29-40 with sound_speed<1> (eos.hpp:469-479, 574-583, 680-689)
+template<typename EP>
+__device__ __forceinline__ double material_sound_speed(const EP& P, int k, double rho, double p)
 {
     const double r = clamp_min(rho);
     const double E = r * mat_internal_energy(P, k, r, p);
     const double r2 = clamp_min(r);
     const double pp = mat_pressure_c1(P, k, r, E);
+    if (EP::kNA && P.kind[k] == EOS_NOBLE_ABEL) {
+        const double denom = safe_denom(r2 * (1.0 - P.covol[k] * r2), 1e-14);
+        return __builtin_sqrt(clamp_min(P.gamma[k] * pp / denom));
+    }
     if (P.kind[k] == EOS_STIFFENED) return __builtin_sqrt(P.gamma[k] * safe_div(pp + P.pinf[k], r2));
     return __builtin_sqrt(P.gamma[k] * safe_div(pp, r2));
 }
 
 // eos.hpp:42-57
-__device__ __forceinline__ double material_energy_pressure_derivative(const EosParams& P, int k,
+template<typename EP>
+__device__ __forceinline__ double material_energy_pressure_derivative(const EP& P, int k,
                                                                       double rho, double p)
 {
     rho = clamp_min(rho);
@@ -116,19 +139,24 @@ __device__ __forceinline__ double material_energy_pressure_derivative(const EosP
 // ---- mixture EoS (src/dim/eos.hpp) ---------------------------------------
 // Loop-invariant part of the residual of the pressure bisection:
 // coef_k = max(finite_or(alpha_k),0) * clamp_min(rho_k), den_k = (gamma_k-1)*clamp_min(rho_k)
-template<int M>
+// (guarded by safe_denom except for noble-Abel, whose internal energy divides
+// unguarded), omb_k = 1 - b_k*clamp_min(rho_k) for noble-Abel materials.
+// NA (EP::kNA of the parameter block) selects the variant that carries omb;
+// EosParams (no noble-Abel material) compiles the residual without it.
+template<int M, bool NA>
 struct MixCoef {
     double coef[M];
     double den[M];
     double rden[M];  // RN(1/den): the residual's divisions by den become div_rn
+    double omb[NA ? M : 1];
 };
 
 // a/b correctly rounded from y = RN(1/b) (Markstein's theorem: q = a*y is
 // within one ulp of a/b, r = a - b*q is exact with an FMA, and RN(q + r*y) is
 // RN(a/b)).  Valid for finite a and normal b with a/b, a*y and r = a - b*q
-// in the normal range, which the bisection's operands satisfy (a >= 1e-10,
-// 1e-10 <= b, a/b < 1e300): three FMA-pipe operations in place of the
-// ten-instruction IEEE division sequence, with the same bits.
+// in the normal range, which the bisection's operands satisfy (|a| >= 1e-10
+// or a = 0, 1e-10*(gamma-1) <= b, |a/b| < 1e300): three FMA-pipe operations in
+// place of the ten-instruction IEEE division sequence, with the same bits.
 __device__ __forceinline__ double div_rn(double a, double b, double y)
 {
     const double q = a * y;
@@ -136,40 +164,81 @@ __device__ __forceinline__ double div_rn(double a, double b, double y)
     return __builtin_fma(r, y, q);
 }
 
-template<int M>
-__device__ __forceinline__ MixCoef<M> mix_coef(const EosParams& P, const double* alpha, const double* rho)
+template<int M, bool NA, typename EP>
+__device__ __forceinline__ MixCoef<M, NA> mix_coef(const EP& P, const double* alpha, const double* rho)
 {
-    MixCoef<M> c;
+    MixCoef<M, NA> c;
 #pragma unroll
     for (int k = 0; k < M; ++k) {
         const double rk = clamp_min(rho[k]);
         c.coef[k] = smax(finite_or(alpha[k]), 0.0) * rk;
-        c.den[k] = safe_denom(P.gm1[k] * rk);
+        if (NA && P.kind[k] == EOS_NOBLE_ABEL) {
+            c.den[k] = P.gm1[k] * rk;
+            c.omb[k] = 1.0 - P.covol[k] * rk;
+        } else {
+            c.den[k] = safe_denom(P.gm1[k] * rk);
+        }
         c.rden[k] = 1.0 / c.den[k];
     }
     return c;
 }
 
+// den_k * e_k(rho_k, p) as the reference forms it before its division, at
+// pk = clamp_min(p): ideal p (guarded), stiffened p + gamma p_inf (guarded),
+// noble-Abel p (1 - b rho_k) (eos.hpp:482-489, 587-595, 692-702)
+template<int M, bool NA, typename EP>
+__device__ __forceinline__ double mix_num(const EP& P, const MixCoef<M, NA>& c, int k, double pk)
+{
+    if (NA && P.kind[k] == EOS_NOBLE_ABEL) return pk * c.omb[NA ? k : 0];
+    return finite_or((P.kind[k] == EOS_STIFFENED) ? (pk + P.gpinf[k]) : pk);
+}
+
 // eos.hpp:59-75 (mixture_internal_energy_density) with the invariants hoisted
-template<int M>
-__device__ __forceinline__ double mix_energy(const EosParams& P, const MixCoef<M>& c, double p)
+template<int M, bool NA, typename EP>
+__device__ __forceinline__ double mix_energy(const EP& P, const MixCoef<M, NA>& c, double p)
 {
     const double pk = clamp_min(p);
     double s = 0.0;
 #pragma unroll
-    for (int k = 0; k < M; ++k) {
-        const double num = (P.kind[k] == EOS_STIFFENED) ? (pk + P.gpinf[k]) : pk;
-        s += c.coef[k] * div_rn(finite_or(num), c.den[k], c.rden[k]);
-    }
+    for (int k = 0; k < M; ++k) s += c.coef[k] * div_rn(mix_num<M, NA>(P, c, k, pk), c.den[k], c.rden[k]);
     return finite_or(s);
+}
+
+// eos.hpp:95-142: bracket doubling from [1e-12, max(1, fallback)], then
+// bisection to |f| <= 1e-10 max(1, rho e)
+template<int M, bool NA, typename EP>
+__device__ __forceinline__ double bisect_pressure(const EP& P, double target, double fallback,
+                                                  const double* alpha, const double* rho)
+{
+    const MixCoef<M, NA> c = mix_coef<M, NA>(P, alpha, rho);
+    double lo = 1e-12;
+    double hi = smax(1.0, fallback);
+    const double flo = mix_energy<M, NA>(P, c, lo) - target;
+    double fhi = mix_energy<M, NA>(P, c, hi) - target;
+    for (int n = 0; n < 80 && finite(fhi) && fhi < 0.0; ++n) {
+        hi *= 2.0;
+        fhi = mix_energy<M, NA>(P, c, hi) - target;
+    }
+    if (!finite(flo) || !finite(fhi) || !(flo <= 0.0 && fhi >= 0.0)) return clamp_min(fallback);
+
+    const double tol = 1e-10 * smax(1.0, target);
+    for (int n = 0; n < 80; ++n) {
+        const double mid = 0.5 * (lo + hi);
+        const double fmid = mix_energy<M, NA>(P, c, mid) - target;
+        if (!finite(fmid)) return clamp_min(fallback);
+        if (__builtin_fabs(fmid) <= tol) return clamp_min(mid);
+        if (fmid > 0.0) hi = mid;
+        else lo = mid;
+    }
+    return clamp_min(0.5 * (lo + hi));
 }
 
 // eos.hpp:77-143 — bracket doubling + bisection, run as the reference runs
 // it; only loop-invariant subexpressions are hoisted (mix_coef), which leaves
 // every rounded value unchanged.  (A replica that skips residuals far from the
 // root measured slower on gfx950: the loop is latency-bound, not residual-bound.)
-template<int M>
-__device__ double pressure_from_internal_energy(const EosParams& P, double rho_total, double e,
+template<int M, typename EP>
+__device__ double pressure_from_internal_energy(const EP& P, double rho_total, double e,
                                                 const double* alpha, const double* rho)
 {
     rho_total = clamp_min(rho_total);
@@ -183,53 +252,43 @@ __device__ double pressure_from_internal_energy(const EosParams& P, double rho_t
                     material_pressure_from_density_energy(P, k, clamp_min(rho[k]), ek);
     }
     fallback = clamp_min(fallback);
+    return bisect_pressure<M, EP::kNA>(P, target, fallback, alpha, rho);
+}
 
-    const MixCoef<M> c = mix_coef<M>(P, alpha, rho);
-    double lo = 1e-12;
-    double hi = smax(1.0, fallback);
-    const double flo = mix_energy<M>(P, c, lo) - target;
-    double fhi = mix_energy<M>(P, c, hi) - target;
-    for (int n = 0; n < 80 && finite(fhi) && fhi < 0.0; ++n) {
-        hi *= 2.0;
-        fhi = mix_energy<M>(P, c, hi) - target;
-    }
-    if (!finite(flo) || !finite(fhi) || !(flo <= 0.0 && fhi >= 0.0)) return clamp_min(fallback);
+// eos.hpp:59-75 at one pressure with plain divisions (same bits as div_rn)
+template<int M, bool NA, typename EP>
+__device__ __forceinline__ double mix_energy_once(const EP& P, double p, const double* alpha,
+                                                  const double* rho)
+{
+    const MixCoef<M, NA> c = mix_coef<M, NA>(P, alpha, rho);
+    const double pk = clamp_min(p);
+    double sum = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) sum += c.coef[k] * (mix_num<M, NA>(P, c, k, pk) / c.den[k]);
+    return finite_or(sum);
+}
 
-    const double tol = 1e-10 * smax(1.0, target);
-    for (int n = 0; n < 80; ++n) {
-        const double mid = 0.5 * (lo + hi);
-        const double fmid = mix_energy<M>(P, c, mid) - target;
-        if (!finite(fmid)) return clamp_min(fallback);
-        if (__builtin_fabs(fmid) <= tol) return clamp_min(mid);
-        if (fmid > 0.0) hi = mid;
-        else lo = mid;
-    }
-    return clamp_min(0.5 * (lo + hi));
+template<int M, typename EP>
+__device__ __forceinline__ double mixture_internal_energy_density(const EP& P, double p,
+                                                                  const double* alpha, const double* rho)
+{
+    return mix_energy_once<M, EP::kNA>(P, p, alpha, rho);
 }
 
 // eos.hpp:163-181 via eos.hpp:59-75
-template<int M>
-__device__ __forceinline__ double internal_energy_from_pressure(const EosParams& P, double rho_total,
+template<int M, typename EP>
+__device__ __forceinline__ double internal_energy_from_pressure(const EP& P, double rho_total,
                                                                 double p, const double* alpha,
                                                                 const double* rho)
 {
     rho_total = clamp_min(rho_total);
     p = clamp_min(p);
-    const MixCoef<M> c = mix_coef<M>(P, alpha, rho);
-    // one evaluation: the plain division (same bits as div_rn, fewer operations here)
-    const double pk = clamp_min(p);
-    double sum = 0.0;
-#pragma unroll
-    for (int k = 0; k < M; ++k) {
-        const double num = (P.kind[k] == EOS_STIFFENED) ? (pk + P.gpinf[k]) : pk;
-        sum += c.coef[k] * (finite_or(num) / c.den[k]);
-    }
-    return finite_or(sum) / rho_total;
+    return mixture_internal_energy_density<M>(P, p, alpha, rho) / rho_total;
 }
 
 // eos.hpp:195-221 (Wood) and eos.hpp:223-275 (allaire / equal_velocity)
-template<int M>
-__device__ double mixture_sound_speed(const EosParams& P, double rho_total, double p,
+template<int M, typename EP>
+__device__ double mixture_sound_speed(const EP& P, double rho_total, double p,
                                       const double* alpha, const double* rho)
 {
     if (P.lambda_model == LAMBDA_KAPILA) {
@@ -314,8 +373,8 @@ __device__ __forceinline__ double total_density(const State<D, M>& U)
 }
 
 // primitives.hpp:111-162
-template<int D, int M>
-__device__ __forceinline__ void cons_to_prim(const EosParams& P, const State<D, M>& U, Prim<D, M>& R)
+template<int D, int M, typename EP>
+__device__ __forceinline__ void cons_to_prim(const EP& P, const State<D, M>& U, Prim<D, M>& R)
 {
     full_alpha<D, M>(U, R.alpha);
     sanitise_alpha<M>(R.alpha, 1e-12);
@@ -338,8 +397,8 @@ __device__ __forceinline__ void cons_to_prim(const EosParams& P, const State<D, 
 }
 
 // primitives.hpp:166-212
-template<int D, int M>
-__device__ __forceinline__ void prim_to_cons(const EosParams& P, const Prim<D, M>& Pin, State<D, M>& U)
+template<int D, int M, typename EP>
+__device__ __forceinline__ void prim_to_cons(const EP& P, const Prim<D, M>& Pin, State<D, M>& U)
 {
     double alpha[M], rho[M];
 #pragma unroll
@@ -367,8 +426,8 @@ __device__ __forceinline__ void prim_to_cons(const EosParams& P, const Prim<D, M
 }
 
 // primitives.hpp:216-322
-template<int D, int M>
-__device__ __forceinline__ void alpha_rhs_coefficients(const EosParams& P, const Prim<D, M>& Pr,
+template<int D, int M, typename EP>
+__device__ __forceinline__ void alpha_rhs_coefficients(const EP& P, const Prim<D, M>& Pr,
                                                        double* rhs)
 {
     const double floor = 1e-12;
@@ -432,8 +491,8 @@ __device__ __forceinline__ void alpha_rhs_coefficients(const EosParams& P, const
 // primitives.hpp:326-457.  The reference runs a pressure bisection
 // (pressure_before) on every call but reads it only when inactive mass is
 // removed; here it is evaluated on that branch only, from the same inputs.
-template<int D, int M>
-__device__ __forceinline__ void repair_state(const EosParams& P, State<D, M>& U)
+template<int D, int M, typename EP>
+__device__ __forceinline__ void repair_state(const EP& P, State<D, M>& U)
 {
     const double mass_floor = 1e-12, p_floor = 1e-12, inactive = 1e-10;
 #pragma unroll
@@ -510,8 +569,7 @@ __device__ __forceinline__ void repair_state(const EosParams& P, State<D, M>& U)
 #pragma unroll
     for (int k = 0; k < M; ++k) rho_m[k] = smax(safe_div(U.m[k], smax(alpha_full[k], inactive)), mass_floor);
     const double kinetic = 0.5 * rho_total * v2;
-    const MixCoef<M> c = mix_coef<M>(P, alpha_full, rho_m);
-    const double minimum_internal = mix_energy<M>(P, c, p_floor);
+    const double minimum_internal = mixture_internal_energy_density<M>(P, p_floor, alpha_full, rho_m);
     U.E = smax(finite_or(U.E), kinetic + minimum_internal);
 }
 
@@ -589,8 +647,8 @@ __device__ __forceinline__ void flux_normal(const State<D, M>& U, const Prim<D, 
 }
 
 // muscl.hpp:143-191
-template<int D, int M, int DIR>
-__device__ __forceinline__ void hancock_predict(const EosParams& P, const State<D, M>& Uc,
+template<int D, int M, int DIR, typename EP>
+__device__ __forceinline__ void hancock_predict(const EP& P, const State<D, M>& Uc,
                                                 const Prim<D, M>& Pc, const State<D, M>& ULf,
                                                 const State<D, M>& URf, const Prim<D, M>& PLf,
                                                 const Prim<D, M>& PRf, const double* rhs, double dt, double dx,
@@ -626,8 +684,8 @@ __device__ __forceinline__ void hancock_predict(const EosParams& P, const State<
 // ---- HLLC (src/dim/riemann/hllc.hpp:15-195) ------------------------------
 // Returns false on the reference's throwing branch (non-finite density or
 // pressure in require_positive, numerical_safety.hpp:67-78).
-template<int D, int M, int DIR>
-__device__ __forceinline__ bool hllc_flux(const EosParams& P, const State<D, M>& UL,
+template<int D, int M, int DIR, typename EP>
+__device__ __forceinline__ bool hllc_flux(const EP& P, const State<D, M>& UL,
                                           const State<D, M>& UR, const Prim<D, M>& PL,
                                           const Prim<D, M>& PR, Flux<D, M>& F, double& fv)
 {

@@ -3,7 +3,7 @@
 
 namespace dimhip {
 
-enum : int { EOS_IDEAL = 0, EOS_STIFFENED = 1 };
+enum : int { EOS_IDEAL = 0, EOS_STIFFENED = 1, EOS_NOBLE_ABEL = 2 };
 enum : int { LAMBDA_KAPILA = 0, LAMBDA_ALLAIRE = 1, LAMBDA_EQUAL_VELOCITY = 2 };
 enum : int { BC_TRANSMISSIVE = 0, BC_REFLECTIVE = 1, BC_NONREFLECTIVE = 2, BC_REFLECTION = 3 };
 
@@ -12,15 +12,26 @@ constexpr int kMaxMat = 4;
 // Material constants.  gm1 = gamma - 1 and gpinf = gamma * p_inf are the
 // reference's own subexpressions ((params.gamma - 1.0), params.gamma *
 // params.p_inf in src/euler/eos.hpp), computed once on the host with the same
-// double operations.
+// double operations.  covol is the noble-Abel co-volume b (params.covolume).
 struct EosParams {
     int kind[kMaxMat];
     double gamma[kMaxMat];
     double pinf[kMaxMat];
     double gm1[kMaxMat];
     double gpinf[kMaxMat];
+    double covol[kMaxMat];
+    int noble_abel;  // nonzero when any material is noble-Abel
     int lambda_model;
     double alpha_source_floor;
+    // EosParams compiles the ideal and stiffened-gas laws only; EosParamsNA
+    // (same layout) adds the noble-Abel branches, chosen per material by kind
+    static constexpr bool kNA = false;
+};
+
+struct EosParamsNA : EosParams {
+    static constexpr bool kNA = true;
+    EosParamsNA() = default;
+    explicit EosParamsNA(const EosParams& p) : EosParams(p) {}
 };
 
 // Single-precision copies for the mixed-precision form.

@@ -14,10 +14,9 @@
 
 #include "../../include/dimhip.h"
 #include "batch.hpp"
+#include "context.hpp"
 #include "ic_kernels.hpp"
-#include "io_kernels.hpp"
 #include "kernels_fp64.hpp"
-#include "policy_fp32.hpp"
 
 using namespace dimhip;
 
@@ -25,62 +24,8 @@ namespace {
 
 thread_local std::string g_global_err;
 
-constexpr int kSweepL0 = 128, kSweepW0 = 2;   // x sweep: 128 positions x 2 lines
-constexpr int kSweepL1 = 64, kSweepW1 = 8;    // y/z sweep: 8 lines (x) x 64 positions
-
-enum KernelClass { KC_SWEEP = 0, KC_BC = 1, KC_CFL = 2, KC_COUNT = 3 };
-
-struct Ops;
-
 }  // namespace
 
-struct dimhip_ctx {
-    dimhip_desc desc{};
-    Grid grid{};
-    EosParams eos{};
-    EosMixed eosm{};
-    BcArgs bc{};
-    int nv = 0;
-    size_t cell_bytes = 0;  // storage bytes per cell
-    hipStream_t stream = nullptr;
-    void* q[2] = {nullptr, nullptr};
-    int cur = 0;
-    StepCtl* ctl = nullptr;
-    StepCtl* ctl_host = nullptr;  // pinned
-    StepCtl* probe = nullptr;     // pinned: step control read back while steps are in flight
-    hipEvent_t probe_event = nullptr;
-    double* staging = nullptr;
-    long long staging_cells = 0;
-    double* partial = nullptr;
-    double* totals = nullptr;
-    int partial_blocks = 0;
-    // driver I/O: primitive rows in, CSV row values out (grown on demand)
-    double* io_buf = nullptr;
-    size_t io_doubles = 0;
-    // boundary flux: per-block partials, last result, running integral
-    long long flux_lines[4] = {0, 0, 0, 0};
-    int flux_blocks = 0;
-    double* flux_partial = nullptr;
-    double* flux_out = nullptr;
-    double* flux_integral = nullptr;
-    bool track_flux = false;
-    bool dt_ready = false;       // ctl->dtmin[slot] holds the CFL minimum of the current state
-    double dt_ready_cfl = 0.0;   // the cfl it was computed with
-    std::string err;
-    const Ops* ops = nullptr;
-    // graph for dimhip_advance
-    hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t graph = nullptr;
-    double graph_cfl = -1.0;
-    int graph_cur = -1;
-    // event timing
-    bool timing = false;
-    std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<int, int>> ev_pending;  // (class, index of start event)
-    size_t ev_next = 0;
-    double kt_sum[KC_COUNT] = {0, 0, 0};
-    long long kt_cnt[KC_COUNT] = {0, 0, 0};
-};
 
 namespace {
 
@@ -137,141 +82,52 @@ void ev_collect(dimhip_ctx* c)
 }
 
 // ---- per-(D, M) launch table ---------------------------------------------
-struct Ops {
-    virtual ~Ops() = default;
-    virtual void sweep(dimhip_ctx* c, int dir, int mode, bool fuse, const void* in, void* out) const = 0;
-    virtual void bc(dimhip_ctx* c, void* q, int end_step) const = 0;
-    virtual void cfl(dimhip_ctx* c, const void* q, bool unsplit, double cfl) const = 0;
-    virtual void totals(dimhip_ctx* c, const void* q) const = 0;
-    virtual void pack(dimhip_ctx* c, bool in, double* staging, void* q, long long first, long long count) const = 0;
-    virtual int kat(const EosParams& P, int which, int dir, int count, const double* in0, const double* in1,
-                    double* out0, double* out1, int* err, hipStream_t s) const = 0;
-    virtual void prim_to_packed(dimhip_ctx* c, const double* prim, double* packed, long long count) const = 0;
-    virtual void output_rows(dimhip_ctx* c, const double* packed, double* rows, long long count) const = 0;
-    virtual void boundary_flux(dimhip_ctx* c, const void* q, double* integral) const = 0;
-    virtual void ic_prims(dimhip_ctx* c, const IcDesc& ic, long long first, long long count, double* prim,
-                          int* err) const = 0;
-};
-
-CflArgs make_cfl_args(const dimhip_ctx* c, double cfl, bool unsplit)
-{
-    CflArgs a{};
-    for (int d = 0; d < 3; ++d) {
-        a.dx[d] = c->grid.dx[d];
-        a.cfl_dx[d] = cfl * c->grid.dx[d];
-        a.inv_cfl_dx[d] = 1.0 / a.cfl_dx[d];
-        a.inv_dx[d] = 1.0 / c->grid.dx[d];
-    }
-    a.cfl = cfl;
-    a.inv_cfl = 1.0 / cfl;
-    a.unsplit = unsplit ? 1 : 0;
-    return a;
-}
-
-int grid_blocks(long long work, int per)
-{
-    long long b = (work + per - 1) / per;
-    if (b < 1) b = 1;
-    if (b > 8192) b = 8192;
-    return (int)b;
-}
-
+// One (dim, nmat): dispatch on the context's precision form (and, in FP64,
+// on whether a material is noble-Abel) to the policy launchers.
 template<int D, int M>
 struct OpsImpl final : Ops {
-    template<typename Pol, int DIR, int MODE, bool FUSE>
-    static void launch_sweep(dimhip_ctx* c, const void* in, void* out, const typename Pol::Eos& P, const CflArgs& ca)
-    {
-        const Grid& g = c->grid;
-        if constexpr (DIR == 0) {
-            constexpr int T = kSweepL0 - 6;
-            dim3 blk(kSweepL0, kSweepW0, 1);
-            dim3 grd((g.N[0] + T - 1) / T, (g.N[1] + kSweepW0 - 1) / kSweepW0, g.N[2]);
-            sweep_kernel<Pol, D, M, 0, kSweepL0, kSweepW0, MODE, FUSE><<<grd, blk, 0, c->stream>>>(in, out, g, P, c->ctl, ca);
-        } else {
-            constexpr int T = kSweepL1 - 6;
-            dim3 blk(kSweepW1, kSweepL1, 1);
-            dim3 grd((g.N[0] + kSweepW1 - 1) / kSweepW1, (g.N[DIR] + T - 1) / T, DIR == 1 ? g.N[2] : g.N[1]);
-            sweep_kernel<Pol, D, M, DIR, kSweepL1, kSweepW1, MODE, FUSE><<<grd, blk, 0, c->stream>>>(in, out, g, P, c->ctl, ca);
-        }
-    }
-
-    template<typename Pol, int DIR>
-    static void sweep_pol(dimhip_ctx* c, int mode, bool fuse, const void* in, void* out, const typename Pol::Eos& P)
-    {
-        const CflArgs ca = make_cfl_args(c, c->dt_ready_cfl, c->desc.time_update == DIMHIP_UNSPLIT);
-        if (mode == 0 && !fuse) launch_sweep<Pol, DIR, 0, false>(c, in, out, P, ca);
-        else if (mode == 0 && fuse) launch_sweep<Pol, DIR, 0, true>(c, in, out, P, ca);
-        else launch_sweep<Pol, DIR, 1, false>(c, in, out, P, ca);
-    }
-
-    template<int DIR>
-    static void sweep_dir(dimhip_ctx* c, int mode, bool fuse, const void* in, void* out)
-    {
-        if (c->desc.precision == DIMHIP_FP64) sweep_pol<Fp64, DIR>(c, mode, fuse, in, out, c->eos);
-        else sweep_pol<Fp32Mixed, DIR>(c, mode, fuse, in, out, c->eosm);
-    }
-
     void sweep(dimhip_ctx* c, int dir, int mode, bool fuse, const void* in, void* out) const override
     {
-        if (dir == 0) sweep_dir<0>(c, mode, fuse, in, out);
-        if constexpr (D >= 2)
-            if (dir == 1) sweep_dir<1>(c, mode, fuse, in, out);
-        if constexpr (D >= 3)
-            if (dir == 2) sweep_dir<2>(c, mode, fuse, in, out);
+        if (c->desc.precision != DIMHIP_FP64) PolOps<Fp32Mixed, D, M>::sweep(c, dir, mode, fuse, in, out, c->eosm);
+        else if (c->eos.noble_abel) PolOps<Fp64NA, D, M>::sweep(c, dir, mode, fuse, in, out, c->eos_na);
+        else PolOps<Fp64, D, M>::sweep(c, dir, mode, fuse, in, out, c->eos);
     }
 
     void bc(dimhip_ctx* c, void* q, int end_step) const override
     {
-        const long long total = c->bc.face_start[2 * D];
-        const int blocks = grid_blocks(total, 256);
-        if (c->desc.precision == DIMHIP_FP64)
-            bc_kernel<Fp64, D, M><<<blocks, 256, 0, c->stream>>>(q, c->grid, c->eos, c->bc, c->ctl, end_step);
-        else
-            bc_kernel<Fp32Mixed, D, M><<<blocks, 256, 0, c->stream>>>(q, c->grid, c->eosm, c->bc, c->ctl, end_step);
+        if (c->desc.precision != DIMHIP_FP64) PolOps<Fp32Mixed, D, M>::bc(c, q, end_step, c->eosm);
+        else if (c->eos.noble_abel) PolOps<Fp64NA, D, M>::bc(c, q, end_step, c->eos_na);
+        else PolOps<Fp64, D, M>::bc(c, q, end_step, c->eos);
     }
 
     void cfl(dimhip_ctx* c, const void* q, bool unsplit, double cfl) const override
     {
-        const CflArgs ca = make_cfl_args(c, cfl, unsplit);
-        const int blocks = grid_blocks(c->grid.ncells, 256 * 4);
-        if (c->desc.precision == DIMHIP_FP64)
-            cfl_kernel<Fp64, D, M><<<blocks, 256, 0, c->stream>>>(q, c->grid, c->eos, c->ctl, ca);
-        else
-            cfl_kernel<Fp32Mixed, D, M><<<blocks, 256, 0, c->stream>>>(q, c->grid, c->eosm, c->ctl, ca);
+        if (c->desc.precision != DIMHIP_FP64) PolOps<Fp32Mixed, D, M>::cfl(c, q, unsplit, cfl, c->eosm);
+        else if (c->eos.noble_abel) PolOps<Fp64NA, D, M>::cfl(c, q, unsplit, cfl, c->eos_na);
+        else PolOps<Fp64, D, M>::cfl(c, q, unsplit, cfl, c->eos);
     }
 
+    // the FP64 totals and packing touch no material law
     void totals(dimhip_ctx* c, const void* q) const override
     {
-        double volume = 1.0;
-        for (int d = 0; d < D; ++d) volume *= c->grid.dx[d];
-        const long long chunk = (c->grid.ncells + c->partial_blocks - 1) / c->partial_blocks;
-        if (c->desc.precision == DIMHIP_FP64)
-            totals_kernel<Fp64, D, M><<<c->partial_blocks, 256, 0, c->stream>>>(q, c->grid, c->eos, volume, chunk, c->partial);
-        else
-            totals_kernel<Fp32Mixed, D, M><<<c->partial_blocks, 256, 0, c->stream>>>(q, c->grid, c->eosm, volume, chunk, c->partial);
-        sum_partials_kernel<<<1, 64, 0, c->stream>>>(c->partial, c->partial_blocks, M + 1 + D + 1, c->totals);
+        if (c->desc.precision != DIMHIP_FP64) PolOps<Fp32Mixed, D, M>::totals(c, q, c->eosm);
+        else PolOps<Fp64, D, M>::totals(c, q, c->eos);
     }
 
     void pack(dimhip_ctx* c, bool in, double* staging, void* q, long long first, long long count) const override
     {
-        const int blocks = grid_blocks(count, 256);
-        if (c->desc.precision == DIMHIP_FP64) {
-            if (in) pack_in_kernel<Fp64, D, M><<<blocks, 256, 0, c->stream>>>(c->eos, staging, q, first, count, c->grid.pitch);
-            else pack_out_kernel<Fp64, D, M><<<blocks, 256, 0, c->stream>>>(c->eos, q, staging, first, count, c->grid.pitch);
-        } else {
-            if (in) pack_in_kernel<Fp32Mixed, D, M><<<blocks, 256, 0, c->stream>>>(c->eosm, staging, q, first, count, c->grid.pitch);
-            else pack_out_kernel<Fp32Mixed, D, M><<<blocks, 256, 0, c->stream>>>(c->eosm, q, staging, first, count, c->grid.pitch);
-        }
+        if (c->desc.precision != DIMHIP_FP64) PolOps<Fp32Mixed, D, M>::pack(c, in, staging, q, first, count, c->eosm);
+        else PolOps<Fp64, D, M>::pack(c, in, staging, q, first, count, c->eos);
     }
 
     void prim_to_packed(dimhip_ctx* c, const double* prim, double* packed, long long count) const override
     {
-        prim_to_packed_kernel<D, M><<<grid_blocks(count, 256), 256, 0, c->stream>>>(c->eos, prim, packed, count);
+        IoOps<D, M>::prim_to_packed(c, prim, packed, count);
     }
 
     void output_rows(dimhip_ctx* c, const double* packed, double* rows, long long count) const override
     {
-        output_rows_kernel<D, M><<<grid_blocks(count, 256), 256, 0, c->stream>>>(c->eos, packed, rows, count);
+        IoOps<D, M>::output_rows(c, packed, rows, count);
     }
 
     void ic_prims(dimhip_ctx* c, const IcDesc& ic, long long first, long long count, double* prim,
@@ -283,36 +139,16 @@ struct OpsImpl final : Ops {
 
     void boundary_flux(dimhip_ctx* c, const void* q, double* integral) const override
     {
-        const long long* L = c->flux_lines;
-        if (c->desc.precision == DIMHIP_FP64)
-            boundary_flux_kernel<Fp64, D, M><<<c->flux_blocks, 256, 0, c->stream>>>(q, c->grid, c->eos, c->eos, L[0],
-                                                                                      L[1], L[2], L[3], c->flux_partial);
-        else
-            boundary_flux_kernel<Fp32Mixed, D, M><<<c->flux_blocks, 256, 0, c->stream>>>(
-                q, c->grid, c->eosm, c->eos, L[0], L[1], L[2], L[3], c->flux_partial);
-        finish_boundary_flux_kernel<<<1, 64, 0, c->stream>>>(c->flux_partial, c->flux_blocks, FluxTotals<D, M>::NO,
-                                                              c->flux_out, integral, c->ctl);
+        if (c->desc.precision != DIMHIP_FP64) PolOps<Fp32Mixed, D, M>::boundary_flux(c, q, integral, c->eosm);
+        else PolOps<Fp64, D, M>::boundary_flux(c, q, integral, c->eos);
     }
 
     int kat(const EosParams& P, int which, int dir, int count, const double* in0, const double* in1, double* out0,
             double* out1, int* err, hipStream_t s) const override
     {
-        const int blocks = (count + 127) / 128;
-        if (dir == 0) {
-            kat64_kernel<D, M, 0><<<blocks, 128, 0, s>>>(P, which, count, in0, in1, out0, out1, err);
-            return 0;
-        }
-        if constexpr (D >= 2)
-            if (dir == 1) {
-                kat64_kernel<D, M, 1><<<blocks, 128, 0, s>>>(P, which, count, in0, in1, out0, out1, err);
-                return 0;
-            }
-        if constexpr (D >= 3)
-            if (dir == 2) {
-                kat64_kernel<D, M, 2><<<blocks, 128, 0, s>>>(P, which, count, in0, in1, out0, out1, err);
-                return 0;
-            }
-        return 1;
+        if (P.noble_abel)
+            return KatOps<D, M, EosParamsNA>::kat(EosParamsNA(P), which, dir, count, in0, in1, out0, out1, err, s);
+        return KatOps<D, M, EosParams>::kat(P, which, dir, count, in0, in1, out0, out1, err, s);
     }
 };
 
@@ -333,14 +169,37 @@ const Ops* select_ops(int dim, int nmat)
     return nullptr;
 }
 
+// Material constants of a descriptor (unused slots: an inert ideal gas).
+// gm1 and gpinf are formed with the reference's own double operations.
+void fill_eos(EosParams& P, const dimhip_desc& d)
+{
+    P = EosParams{};
+    for (int k = 0; k < kMaxMat; ++k) {
+        const bool on = k < d.nmat;
+        P.kind[k] = on ? d.eos_kind[k] : 0;
+        P.gamma[k] = on ? d.gamma[k] : 1.4;
+        P.pinf[k] = on ? d.p_inf[k] : 0.0;
+        P.covol[k] = (on && P.kind[k] == EOS_NOBLE_ABEL) ? d.covolume[k] : 0.0;
+        P.gm1[k] = P.gamma[k] - 1.0;
+        P.gpinf[k] = P.gamma[k] * P.pinf[k];
+        if (on && P.kind[k] == EOS_NOBLE_ABEL) P.noble_abel = 1;
+    }
+    P.lambda_model = d.lambda_model;
+    P.alpha_source_floor = d.alpha_source_floor;
+}
+
 int validate_desc(const dimhip_desc* d, std::string& msg)
 {
     if (d->dim < 1 || d->dim > 3) return msg = "dimhip: dim must be 1, 2 or 3", 1;
     if (d->nmat < 1 || d->nmat > 3) return msg = "dimhip: nmat must be 1, 2 or 3", 1;
     for (int k = 0; k < d->nmat; ++k) {
         if (!(d->gamma[k] > 1.0)) return msg = "dim::EOSParams: gamma must be > 1", 1;
-        if (d->eos_kind[k] != DIMHIP_EOS_IDEAL && d->eos_kind[k] != DIMHIP_EOS_STIFFENED)
+        if (d->eos_kind[k] != DIMHIP_EOS_IDEAL && d->eos_kind[k] != DIMHIP_EOS_STIFFENED &&
+            d->eos_kind[k] != DIMHIP_EOS_NOBLE_ABEL)
             return msg = "dimhip: unsupported EOS kind", 1;
+        if (d->eos_kind[k] == DIMHIP_EOS_NOBLE_ABEL && d->precision == DIMHIP_FP32_MIXED)
+            return msg = "dimhip: the FP32-mixed form supports ideal and stiffened gases only; "
+                         "use DIMHIP_FP64 for noble-Abel materials", 1;
     }
     for (int a = 0; a < d->dim; ++a) {
         if (d->N[a] <= 0) return msg = "dim::total_cells: grid size must be positive", 1;
@@ -544,13 +403,9 @@ int dimhip_create(const dimhip_desc* desc, dimhip_ctx** out)
     g.ncells = (long long)g.N[0] * g.N[1] * g.N[2];
     g.pitch = (g.ncells + 63) / 64 * 64;
     c->nv = 2 * desc->nmat + desc->dim;
+    fill_eos(c->eos, *desc);
+    c->eos_na = EosParamsNA(c->eos);
     for (int k = 0; k < kMaxMat; ++k) {
-        const bool on = k < desc->nmat;
-        c->eos.kind[k] = on ? desc->eos_kind[k] : 0;
-        c->eos.gamma[k] = on ? desc->gamma[k] : 1.4;
-        c->eos.pinf[k] = on ? desc->p_inf[k] : 0.0;
-        c->eos.gm1[k] = c->eos.gamma[k] - 1.0;
-        c->eos.gpinf[k] = c->eos.gamma[k] * c->eos.pinf[k];
         c->eosm.kind[k] = c->eos.kind[k];
         c->eosm.gamma[k] = (float)c->eos.gamma[k];
         c->eosm.pinf[k] = (float)(c->eos.kind[k] == EOS_STIFFENED ? c->eos.pinf[k] : 0.0);
@@ -561,8 +416,6 @@ int dimhip_create(const dimhip_desc* desc, dimhip_ctx** out)
         c->eosm.b[k] = (float)c->eosm.b_d[k];
         c->eosm.bmin[k] = (float)(1e-10 / c->eos.gm1[k]);
     }
-    c->eos.lambda_model = desc->lambda_model;
-    c->eos.alpha_source_floor = desc->alpha_source_floor;
     c->eosm.lambda_model = desc->lambda_model;
     c->eosm.stiff = 0;
     for (int k = 0; k < desc->nmat; ++k)
@@ -1060,16 +913,7 @@ int dimhip_kat(const dimhip_desc* desc, int which, int dir, int count, const dou
     if (which == 3 && (dir < 0 || dir >= D)) return fail(nullptr, "dimhip_kat: direction out of range");
     HIP_OK(nullptr, hipSetDevice(d.device));
     EosParams P{};
-    for (int k = 0; k < kMaxMat; ++k) {
-        const bool on = k < M;
-        P.kind[k] = on ? d.eos_kind[k] : 0;
-        P.gamma[k] = on ? d.gamma[k] : 1.4;
-        P.pinf[k] = on ? d.p_inf[k] : 0.0;
-        P.gm1[k] = P.gamma[k] - 1.0;
-        P.gpinf[k] = P.gamma[k] * P.pinf[k];
-    }
-    P.lambda_model = d.lambda_model;
-    P.alpha_source_floor = d.alpha_source_floor;
+    fill_eos(P, d);
     double *di0 = nullptr, *di1 = nullptr, *do0 = nullptr, *do1 = nullptr;
     int* derr = nullptr;
     auto cleanup = [&] {

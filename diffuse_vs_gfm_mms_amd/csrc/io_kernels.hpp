@@ -14,7 +14,7 @@ namespace dimhip {
 // prim_to_cons(P) (src/setup/dim_initial_conditions.hpp:531-628).
 // prim rows: rho[M], alpha[M], vel[D], p  ->  packed state rows.
 template<int D, int M>
-__global__ void prim_to_packed_kernel(EosParams P, const double* __restrict__ prim, double* __restrict__ packed,
+__global__ void prim_to_packed_kernel(EosParamsNA P, const double* __restrict__ prim, double* __restrict__ packed,
                                       long long count)
 {
     using LY = Layout<D, M>;
@@ -38,7 +38,7 @@ struct RowLayout {
 // P = cons_to_prim(U), rho = total_density(U), e = max(safe_div(E - K, rho), 0)
 // with K = 0.5 * rho * sum(P.vel^2).
 template<int D, int M>
-__global__ void output_rows_kernel(EosParams P, const double* __restrict__ packed, double* __restrict__ rows,
+__global__ void output_rows_kernel(EosParamsNA P, const double* __restrict__ packed, double* __restrict__ rows,
                                    long long count)
 {
     using LY = Layout<D, M>;
@@ -78,7 +78,7 @@ struct FluxTotals {
 };
 
 template<int D, int M, int DIR>
-__device__ __forceinline__ void add_scaled_flux(double* acc, const State<D, M>& U, const EosParams& P, double scale)
+__device__ __forceinline__ void add_scaled_flux(double* acc, const State<D, M>& U, const EosParamsNA& P, double scale)
 {
     Prim<D, M> Q;
     Flux<D, M> F;
@@ -115,7 +115,7 @@ __device__ __forceinline__ void load_fp64_cell(const void* q, long long pitch, l
 // sums its lines in a fixed tree and writes partial[b].
 template<typename Pol, int D, int M>
 __global__ void __launch_bounds__(256)
-boundary_flux_kernel(const void* __restrict__ q, Grid g, typename Pol::Eos PE, EosParams P, long long line_start0,
+boundary_flux_kernel(const void* __restrict__ q, Grid g, typename Pol::Eos PE, EosParamsNA P, long long line_start0,
                      long long line_start1, long long line_start2, long long line_start3, double* __restrict__ partial)
 {
     constexpr int NO = FluxTotals<D, M>::NO;
@@ -171,7 +171,7 @@ boundary_flux_kernel(const void* __restrict__ q, Grid g, typename Pol::Eos PE, E
 // Sums the block partials in order into out.  With integral != nullptr it
 // also performs the report's accumulate_fluxes (conservation_report.hpp:506-520):
 // integral += dt * flux, for a step that was taken (ctl->active).
-__global__ void finish_boundary_flux_kernel(const double* __restrict__ partial, int nblocks, int no,
+static __global__ void finish_boundary_flux_kernel(const double* __restrict__ partial, int nblocks, int no,
                                             double* __restrict__ out, double* __restrict__ integral,
                                             const StepCtl* __restrict__ ctl)
 {

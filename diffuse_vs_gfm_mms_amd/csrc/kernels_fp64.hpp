@@ -18,7 +18,7 @@ struct Layout {
 // parity's dt slots (min is exact in any order); dt = min(dt_max, minimum)
 // exactly as the reference's running std::min from dt = dt_max
 // (cfl.hpp:43-63); the next parity's slots are cleared.
-__global__ void __launch_bounds__(64) begin_step_kernel(StepCtl* ctl, double dt_max_host)
+static __global__ void __launch_bounds__(64) begin_step_kernel(StepCtl* ctl, double dt_max_host)
 {
     const int s = ctl->slot;
     const int k = threadIdx.x;
@@ -46,7 +46,7 @@ __global__ void __launch_bounds__(64) begin_step_kernel(StepCtl* ctl, double dt_
 }
 
 // 64 threads: clear the dt slots of parity slot ^ which_rel
-__global__ void __launch_bounds__(64) reset_dtmin_kernel(StepCtl* ctl, int which_rel)
+static __global__ void __launch_bounds__(64) reset_dtmin_kernel(StepCtl* ctl, int which_rel)
 {
     const int s = ctl->slot ^ which_rel;
     dt_slots(ctl, s)[threadIdx.x * kDtStride] = kInfBits;
@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(64) reset_dtmin_kernel(StepCtl* ctl, int which
 }
 
 // 64 threads: StepCtl::dtmin[slot] = min over the current parity's dt slots
-__global__ void __launch_bounds__(64) reduce_dtmin_kernel(StepCtl* ctl)
+static __global__ void __launch_bounds__(64) reduce_dtmin_kernel(StepCtl* ctl)
 {
     const int s = ctl->slot;
     double m = bits_to_double(dt_slots(ctl, s)[threadIdx.x * kDtStride]);
@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(64) reduce_dtmin_kernel(StepCtl* ctl)
     if (threadIdx.x == 0) ctl->dtmin[s] = double_to_bits(m);
 }
 
-__global__ void set_active_kernel(StepCtl* ctl, double dt)
+static __global__ void set_active_kernel(StepCtl* ctl, double dt)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         ctl->dt = dt;
@@ -124,8 +124,10 @@ __device__ __forceinline__ void store_packed_prim(double* p, const Prim<D, M>& Q
     p[o++] = Q.p;
 }
 
-template<int D, int M, int DIR>
-__global__ void kat64_kernel(EosParams P, int which, int count, const double* __restrict__ in0,
+// EP: EosParams, or EosParamsNA when a material is noble-Abel (the sweeps'
+// own choice, so each parameter block's device functions are the ones tested)
+template<int D, int M, int DIR, typename EP>
+__global__ void kat64_kernel(EP P, int which, int count, const double* __restrict__ in0,
                              const double* __restrict__ in1, double* __restrict__ out0, double* __restrict__ out1,
                              int* __restrict__ err)
 {
@@ -182,7 +184,7 @@ __global__ void kat64_kernel(EosParams P, int which, int count, const double* __
     }
 }
 
-__global__ void kat_ieee_kernel(int count, const double* __restrict__ in, double* __restrict__ div_out,
+static __global__ void kat_ieee_kernel(int count, const double* __restrict__ in, double* __restrict__ div_out,
                                 double* __restrict__ sqrt_out)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;

@@ -737,7 +737,11 @@ __device__ __forceinline__ bool hllc(const EosMixed& P, const St<D, M>& UL, cons
 }  // namespace f32
 
 struct Fp32Mixed {
-    static constexpr int kMinWavesPerEU = 8;
+    // register cap of the sweep kernels: 8 waves per SIMD (64 VGPRs) fits the
+    // 1D/2D two-material sweeps without spills; the 3D and three-material
+    // sweeps need more registers and run uncapped (a cap spills them)
+    template<int D, int M>
+    static constexpr int min_waves_per_eu() { return (D <= 2 && M <= 2) ? 8 : 1; }
     static constexpr bool kReloadInput = false;
     // CFL reduction: each cell contributes its wave rate max_d (|u_d|+c)/(cfl dx_d),
     // combined by max in FP64; the block's dt is 1/rate (reciprocal is monotone,

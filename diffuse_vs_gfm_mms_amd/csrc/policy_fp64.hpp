@@ -3,6 +3,8 @@
 // SoA planes in the reference's field order.
 #pragma once
 
+#include <type_traits>
+
 #include "dim_fp64.hpp"
 
 namespace dimhip {
@@ -21,15 +23,20 @@ constexpr unsigned long long kInfBits = 0x7FF0000000000000ull;
 __device__ __forceinline__ double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
 __device__ __forceinline__ unsigned long long double_to_bits(double x) { return (unsigned long long)__double_as_longlong(x); }
 
-struct Fp64 {
-    static constexpr int kMinWavesPerEU = 4;
+// NA selects the parameter block: EosParams (ideal and stiffened gases) or
+// EosParamsNA (also noble-Abel); Fp64 and Fp64NA below.
+template<bool NA>
+struct Fp64T {
+    // register cap of the sweep kernels: 4 waves per SIMD (128 VGPRs)
+    template<int D, int M>
+    static constexpr int min_waves_per_eu() { return 4; }
     // the sweep re-reads a cell's input state instead of keeping it in
     // registers across its phases (register pressure, not bandwidth, binds here)
     static constexpr bool kReloadInput = true;
     // CFL reduction: each cell contributes its dt candidate, combined by min
     static constexpr bool kCflRate = false;
     using R = double;
-    using Eos = EosParams;
+    using Eos = std::conditional_t<NA, EosParamsNA, EosParams>;
     template<int D, int M> using St = State<D, M>;
     template<int D, int M> using Pr = Prim<D, M>;
     template<int D, int M> using Fx = Flux<D, M>;
@@ -242,5 +249,8 @@ struct Fp64 {
     template<int D, int M>
     static __device__ __forceinline__ double energy(const Eos&, const Cell<D, M>& U) { return U.E; }
 };
+
+using Fp64 = Fp64T<false>;
+using Fp64NA = Fp64T<true>;
 
 }  // namespace dimhip

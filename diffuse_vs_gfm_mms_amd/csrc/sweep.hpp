@@ -423,11 +423,11 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
 }
 
 // one tile per block, blocks in XCD-aware order
-// Pol::kMinWavesPerEU caps the register budget: the FP64 sweep's divide and
+// Pol::min_waves_per_eu caps the register budget: the FP64 sweep's divide and
 // bisection chains are latency-bound, and 4 waves per SIMD (128 VGPRs, a few
 // spills) beat 3 unconstrained waves by ~9%.
 template<typename Pol, int D, int M, int DIR, int L, int W, int MODE, bool FUSE_CFL>
-__global__ void __launch_bounds__(L* W) __attribute__((amdgpu_waves_per_eu(Pol::kMinWavesPerEU)))
+__global__ void __launch_bounds__(L* W) __attribute__((amdgpu_waves_per_eu(Pol::template min_waves_per_eu<D, M>())))
 sweep_kernel(const void* __restrict__ qin, void* __restrict__ qout, Grid g, typename Pol::Eos P,
              StepCtl* __restrict__ ctl, CflArgs cflargs)
 {
@@ -479,6 +479,7 @@ __device__ __forceinline__ void nonreflective_state(const typename Pol::Eos& P, 
 #pragma unroll
     for (int k = 0; k < M - 1; ++k) ex.a[k] = first.a[k] + (first.a[k] - sec.a[k]);
     int axis = 0;
+#pragma unroll
     for (int d = D - 1; d >= 0; --d)
         if (idx[d] == 0 || idx[d] == g.N[d] - 1) axis = d;
     const double outward = (idx[axis] == 0) ? -1.0 : 1.0;
@@ -517,14 +518,25 @@ __device__ __forceinline__ void bc_cell(void* __restrict__ q, const Grid& g, con
 {
     using Cell = typename Pol::template Cell<D, M>;
     {
+        // every array index below is a compile-time constant after unrolling,
+        // so nothing is staged through private (scratch) memory
         int face = 0;
-        while (face < 2 * D - 1 && t >= bc.face_start[face + 1]) ++face;
-        long long local = t - bc.face_start[face];
+        long long base = 0;
+#pragma unroll
+        for (int f = 1; f < 2 * D; ++f)
+            if (t >= bc.face_start[f]) {
+                face = f;
+                base = bc.face_start[f];
+            }
+        long long local = t - base;
         const int fd = face >> 1, side = face & 1;
         int idx[3] = {0, 0, 0};
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            if (d == fd) continue;
+            if (d == fd) {
+                idx[d] = side ? g.N[d] - 1 : 0;
+                continue;
+            }
             int lo = 0, cnt = g.N[d];
             if (d < fd && g.N[d] > 1) {
                 lo = 1;
@@ -533,7 +545,6 @@ __device__ __forceinline__ void bc_cell(void* __restrict__ q, const Grid& g, con
             idx[d] = lo + (int)(local % cnt);
             local /= cnt;
         }
-        idx[fd] = side ? g.N[fd] - 1 : 0;
         int src[3] = {0, 0, 0};
 #pragma unroll
         for (int d = 0; d < D; ++d) {
@@ -542,16 +553,12 @@ __device__ __forceinline__ void bc_cell(void* __restrict__ q, const Grid& g, con
             if (idx[d] == 0) src[d] = 1;
             else if (idx[d] == g.N[d] - 1) src[d] = g.N[d] - 2;
         }
+        // the first axis on which the cell touches the boundary names it
         int name = BC_TRANSMISSIVE;
-        for (int d = 0; d < D; ++d) {
-            if (idx[d] == 0) {
-                name = bc.bc_lo[d];
-                break;
-            }
-            if (idx[d] == g.N[d] - 1) {
-                name = bc.bc_hi[d];
-                break;
-            }
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d) {
+            if (idx[d] == 0) name = bc.bc_lo[d];
+            else if (idx[d] == g.N[d] - 1) name = bc.bc_hi[d];
         }
         long long lin = 0, sl = 0;
 #pragma unroll
@@ -683,7 +690,7 @@ totals_kernel(const void* __restrict__ q, Grid g, typename Pol::Eos P, double vo
         for (int o = 0; o < NO; ++o) partial[(long long)blockIdx.x * NO + o] = red[o][0];
 }
 
-__global__ void sum_partials_kernel(const double* __restrict__ partial, int nblocks, int no, double* __restrict__ out)
+static __global__ void sum_partials_kernel(const double* __restrict__ partial, int nblocks, int no, double* __restrict__ out)
 {
     const int o = threadIdx.x;
     if (o >= no) return;

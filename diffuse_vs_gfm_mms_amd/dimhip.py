@@ -19,7 +19,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "libdimhip.so")
 
-EOS = {"ideal_gas": 0, "stiffened_gas": 1}
+EOS = {"ideal_gas": 0, "stiffened_gas": 1, "noble_abel": 2, "noble_abel_gas": 2, "real_gas": 2}
 BC = {"transmissive": 0, "zero_gradient": 0, "outflow": 0, "reflective": 1,
       "nonreflective": 2, "non_reflective": 2, "reflection": 3}
 LAMBDA = {"kapila": 0, "": 0, "allaire": 1, "equal_velocity": 2}
@@ -45,6 +45,7 @@ class Desc(C.Structure):
         ("time_update", C.c_int),
         ("precision", C.c_int),
         ("device", C.c_int),
+        ("covolume", C.c_double * 4),
     ]
 
 
@@ -209,10 +210,12 @@ def make_desc(dim, N, dx, materials, bc_lo, bc_hi, lambda_model="kapila", alpha_
         d.bc_lo[a] = BC[bc_lo[a]] if a < dim else 0
         d.bc_hi[a] = BC[bc_hi[a]] if a < dim else 0
     d.nmat = len(materials)
-    for k, (kind, g, pinf) in enumerate(materials):
+    for k, mat in enumerate(materials):
+        kind, g, pinf = mat[:3]
         d.eos_kind[k] = EOS[kind]
         d.gamma[k] = g
         d.p_inf[k] = pinf
+        d.covolume[k] = mat[3] if len(mat) > 3 else 0.0
     d.lambda_model = LAMBDA[lambda_model]
     d.alpha_source_floor = alpha_source_floor
     d.time_update = TIME_UPDATE[time_update]

@@ -100,13 +100,14 @@ dimhip_desc make_desc(const CaseConfig<DIM>& c, const std::array<int, DIM>& N, i
     d.nmat = nmat;
     for (const auto& m : c.materials) {
         if (m.id < 0 || m.id >= nmat) throw std::runtime_error("build_dim_material_params: invalid material id");
+        const bool noble_abel = m.type == "real_gas" || m.type == "noble_abel" || m.type == "noble_abel_gas";
         if (m.type == "ideal_gas") d.eos_kind[m.id] = DIMHIP_EOS_IDEAL;
         else if (m.type == "stiffened_gas") d.eos_kind[m.id] = DIMHIP_EOS_STIFFENED;
-        else if (m.type == "real_gas" || m.type == "noble_abel" || m.type == "noble_abel_gas" ||
-                 m.type == "peng_robinson" || m.type == "peng-robinson" || m.type == "tait" ||
+        else if (noble_abel) d.eos_kind[m.id] = DIMHIP_EOS_NOBLE_ABEL;
+        else if (m.type == "peng_robinson" || m.type == "peng-robinson" || m.type == "tait" ||
                  m.type == "mie_gruneisen" || m.type == "mie-gruneisen" || m.type == "mie_gruneisen_fluid")
             throw std::runtime_error("dimhip: EOS type " + m.type +
-                                     " is not supported on the GPU path (ideal_gas, stiffened_gas)");
+                                     " is not supported on the GPU path (ideal_gas, stiffened_gas, noble_abel)");
         else throw std::runtime_error("Unsupported EOS type: " + m.type);
         const auto g = m.params.find("gamma");
         if (g == m.params.end()) throw std::runtime_error("build_material_params: missing gamma in material");
@@ -117,6 +118,12 @@ dimhip_desc make_desc(const CaseConfig<DIM>& c, const std::array<int, DIM>& N, i
             if (p == m.params.end())
                 throw std::runtime_error("build_material_params: missing p_inf in stiffened gas material");
             d.p_inf[m.id] = p->second;
+        }
+        // co-volume: key b, else covolume, else 0 (material_builder.hpp:31-38)
+        d.covolume[m.id] = 0.0;
+        if (noble_abel) {
+            if (m.params.count("b")) d.covolume[m.id] = m.params.at("b");
+            else if (m.params.count("covolume")) d.covolume[m.id] = m.params.at("covolume");
         }
     }
     d.lambda_model = c.dim_lambda_model == "allaire"          ? DIMHIP_LAMBDA_ALLAIRE

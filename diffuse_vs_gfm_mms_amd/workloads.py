@@ -22,6 +22,20 @@ import numpy as np
 WATER = ("stiffened_gas", 4.4, 6.0e8)
 AIR = ("ideal_gas", 1.4, 0.0)
 HELIUM = ("ideal_gas", 1.648, 0.0)
+# noble-Abel air with co-volume b = 1e-3 m^3/kg
+# (configs/DIM/bubble_collapse/gorsse_tc9_water_air_bubble_2d.txt:33-34)
+NA_AIR = ("noble_abel", 1.4, 0.0, 1.0e-3)
+
+
+def material_energy_density(mat, alpha, rho, p):
+    """alpha_k rho_k e_k(rho_k, p) of one material: ideal, stiffened
+    (eos.hpp:482-489, 587-595) or noble-Abel (eos.hpp:692-702)."""
+    kind, g, pinf = mat[:3]
+    if kind in ("noble_abel", "noble_abel_gas", "real_gas"):
+        b = mat[3] if len(mat) > 3 else 0.0
+        return alpha * p * (1.0 - b * rho) / (g - 1.0)
+    add = g * pinf if kind == "stiffened_gas" else 0.0
+    return alpha * (p + add) / (g - 1.0)
 
 
 @dataclasses.dataclass
@@ -60,9 +74,8 @@ def prim_to_packed(materials, alpha, rho, vel, p):
     m = alpha * rho
     rt = m.sum(axis=1)
     e = np.zeros(n)
-    for k, (kind, g, pinf) in enumerate(materials):
-        add = g * pinf if kind == "stiffened_gas" else 0.0
-        e += alpha[:, k] * (p + add) / (g - 1.0)
+    for k, mat in enumerate(materials):
+        e += material_energy_density(mat, alpha[:, k], rho[:, k], p)
     U = np.zeros((n, 2 * nmat + dim))
     U[:, :nmat] = m
     U[:, nmat:nmat + dim] = rt[:, None] * vel
@@ -77,7 +90,7 @@ def _centres(N, lo, dx):
     return [g.reshape(-1) for g in grids[::-1]]  # x-fastest flattening
 
 
-def water_air_tube_1d(n=1000, alpha_floor=1e-6):
+def water_air_tube_1d(n=1000, alpha_floor=1e-6, air=AIR):
     """Two-material shock tube on [0,1], interface at 0.5: water (stiffened gas,
     gamma=4.4, p_inf=6e8, rho=1000, p=1e9) left, air (ideal, gamma=1.4, rho=50,
     p=1e5) right, both at rest, volume fractions floored at alpha_floor."""
@@ -89,9 +102,16 @@ def water_air_tube_1d(n=1000, alpha_floor=1e-6):
     rho = np.stack([np.full(n, 1000.0), np.full(n, 50.0)], axis=1)
     p = np.where(left, 1.0e9, 1.0e5)
     vel = np.zeros((n, 1))
-    mats = [WATER, AIR]
+    mats = [WATER, air]
     U = prim_to_packed(mats, alpha, rho, vel, p)
     return Case("water_air_tube_1d", 1, (n,), (dx,), mats, ("transmissive",), ("transmissive",), U)
+
+
+def water_na_air_tube_1d(n=1000, alpha_floor=1e-6):
+    """The water/air tube of water_air_tube_1d with noble-Abel air (b = 1e-3)."""
+    c = water_air_tube_1d(n, alpha_floor, air=NA_AIR)
+    c.name = "water_na_air_tube_1d"
+    return c
 
 
 def rankine_hugoniot(gamma, mach, rho1, p1):
@@ -209,6 +229,24 @@ def bubble_collapse_3d(n=512, build_state=True):
                         bubble=(1.0, (0.0, 0.0, 0.0), p1), centre=(0.5, 0.5, 0.5), radius=0.15,
                         shock_axis=0, shock_position=0.2, thickness=2.0 * h,
                         name=f"bubble_collapse_3d_{n}", build_state=build_state)
+    return case
+
+
+def gorsse_tc9_2d(nx=480, ny=400, build_state=True):
+    """Gorsse et al. TC9 water/air bubble collapse
+    (configs/DIM/bubble_collapse/gorsse_tc9_water_air_bubble_2d.txt):
+    [-0.2, 1.0] x [0, 1]; stiffened-gas water (gamma 4.4, p_inf 6e8) at
+    rho 1000, p 1e5 left of x = 0.75 and rho 1230, u = -432.69, p 1e9 right of
+    it; a noble-Abel air bubble (gamma 1.4, b = 1e-3) of radius 0.2 at
+    (0.5, 0.5), rho 1.2, p 1e5; interface thickness 0.015; CFL 0.45;
+    zero-gradient x and reflective y boundaries."""
+    case = shock_bubble((nx, ny), (-0.2, 0.0), (1.0, 1.0), [WATER, NA_AIR], 0, 1,
+                        pre=(1000.0, (0.0, 0.0), 1.0e5), post=(1230.0, (-432.69, 0.0), 1.0e9),
+                        bubble=(1.2, (0.0, 0.0), 1.0e5), centre=(0.5, 0.5), radius=0.2,
+                        shock_axis=0, shock_position=0.75, thickness=0.015,
+                        bc_lo=("transmissive", "reflective"), bc_hi=("transmissive", "reflective"),
+                        name=f"gorsse_tc9_{nx}x{ny}", build_state=build_state)
+    case.cfl = 0.45
     return case
 
 

@@ -31,11 +31,13 @@
 extern "C" {
 #endif
 
-#define DIMHIP_VERSION 1
+#define DIMHIP_VERSION 2
 #define DIMHIP_MAX_MAT 4
 
-/* EoS kinds on the path (src/euler/eos_params.hpp:6-13) */
-enum { DIMHIP_EOS_IDEAL = 0, DIMHIP_EOS_STIFFENED = 1 };
+/* EoS kinds on the path (src/euler/eos_params.hpp:6-13; the config names
+ * real_gas / noble_abel / noble_abel_gas select NOBLE_ABEL,
+ * eos_params.hpp:26-28).  NOBLE_ABEL runs in the FP64 form only. */
+enum { DIMHIP_EOS_IDEAL = 0, DIMHIP_EOS_STIFFENED = 1, DIMHIP_EOS_NOBLE_ABEL = 2 };
 
 /* Boundary names (config_loader.hpp:369-377; boundary.hpp:108-318).
  * transmissive / zero_gradient / outflow map to TRANSMISSIVE; the literal
@@ -74,6 +76,9 @@ typedef struct {
     int time_update;                /* DIMHIP_SPLIT / DIMHIP_UNSPLIT */
     int precision;                  /* DIMHIP_FP64 / DIMHIP_FP32_MIXED */
     int device;                     /* HIP device ordinal */
+    double covolume[DIMHIP_MAX_MAT]; /* noble-Abel co-volume b (EOSParams::covolume,
+                                        src/euler/eos_params.hpp:56-57); read for
+                                        NOBLE_ABEL materials only */
 } dimhip_desc;
 
 /* Phase timings of one step, seconds (src/core/phase_timings.hpp:4-21). */

@@ -86,13 +86,20 @@ typedef struct {
     double E;
 } fx_t;
 
-/* IdealGasEOS::pressure / StiffenedGasEOS::pressure on a Conserved<1> with
- * zero momentum (eos.hpp:451-466, 555-571). */
+/* IdealGasEOS::pressure / StiffenedGasEOS::pressure / NobleAbelGasEOS::pressure
+ * on a Conserved<1> with zero momentum (eos.hpp:451-466, 555-571, 665-677;
+ * the noble-Abel internal energy is internal_energy_from_conserved,
+ * eos.hpp:13-25). */
 static double mat_pressure_c1(const orc_params* P, int k, double rho_in, double E)
 {
     const double rho = clamp_min(rho_in);
     const double mom2 = 0.0 + 0.0 * 0.0;
     const double kinetic = 0.5 * safe_div(mom2, rho, 1e-10);
+    if (P->kind[k] == ORC_EOS_NOBLE_ABEL) {
+        const double e = smax(safe_div(E - kinetic, rho, 1e-10), 0.0);
+        const double denom = safe_denom(1.0 - P->covolume[k] * rho, 1e-14);
+        return clamp_min((P->gamma[k] - 1.0) * rho * e / denom);
+    }
     if (P->kind[k] == ORC_EOS_STIFFENED) {
         const double p_raw = (P->gamma[k] - 1.0) * (E - kinetic) - P->gamma[k] * P->pinf[k];
         return clamp_min(p_raw);
@@ -101,19 +108,29 @@ static double mat_pressure_c1(const orc_params* P, int k, double rho_in, double 
     return clamp_min(p_raw);
 }
 
-/* internal_energy (eos.hpp:482-489, 587-595) */
+/* internal_energy (eos.hpp:482-489, 587-595, 692-702); the noble-Abel form
+ * clamps rho and p and divides without a guard */
 static double mat_internal_energy(const orc_params* P, int k, double rho, double p)
 {
+    if (P->kind[k] == ORC_EOS_NOBLE_ABEL) {
+        const double r = clamp_min(rho);
+        const double pp = clamp_min(p);
+        return pp * (1.0 - P->covolume[k] * r) / ((P->gamma[k] - 1.0) * r);
+    }
     if (P->kind[k] == ORC_EOS_STIFFENED)
         return safe_div((p + P->gamma[k] * P->pinf[k]), ((P->gamma[k] - 1.0) * rho), 1e-10);
     return safe_div(p, ((P->gamma[k] - 1.0) * rho), 1e-10);
 }
 
-/* sound_speed on a Conserved<1> (eos.hpp:469-479, 574-583) */
+/* sound_speed on a Conserved<1> (eos.hpp:469-479, 574-583, 680-689) */
 static double mat_sound_speed_c1(const orc_params* P, int k, double rho_in, double E)
 {
     const double rho = clamp_min(rho_in);
     const double p = mat_pressure_c1(P, k, rho_in, E);
+    if (P->kind[k] == ORC_EOS_NOBLE_ABEL) {
+        const double denom = safe_denom(rho * (1.0 - P->covolume[k] * rho), 1e-14);
+        return sqrt(clamp_min(P->gamma[k] * p / denom));
+    }
     if (P->kind[k] == ORC_EOS_STIFFENED)
         return sqrt(P->gamma[k] * safe_div((p + P->pinf[k]), rho, 1e-10));
     return sqrt(P->gamma[k] * safe_div(p, rho, 1e-10));

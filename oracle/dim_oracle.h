@@ -28,7 +28,7 @@ extern "C" {
 #define ORC_MAX_MAT 4
 #define ORC_MAX_DIM 3
 
-enum { ORC_EOS_IDEAL = 0, ORC_EOS_STIFFENED = 1 };
+enum { ORC_EOS_IDEAL = 0, ORC_EOS_STIFFENED = 1, ORC_EOS_NOBLE_ABEL = 2 };
 enum { ORC_LAMBDA_KAPILA = 0, ORC_LAMBDA_ALLAIRE = 1, ORC_LAMBDA_EQUAL_VELOCITY = 2 };
 /* boundary names collapse to three behaviours (boundary.hpp:108-318):
  * copy (transmissive, zero_gradient, outflow, reflection), the literal
@@ -43,6 +43,7 @@ typedef struct {
     double pinf[ORC_MAX_MAT];
     int lambda_model;
     double alpha_source_floor;
+    double covolume[ORC_MAX_MAT];  /* noble-Abel b; read for ORC_EOS_NOBLE_ABEL only */
 } orc_params;
 
 int orc_nv(const orc_params* p);

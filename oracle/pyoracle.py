@@ -21,7 +21,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_LIB = os.path.join(HERE, "liboracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libdimref.so")
 
-EOS_IDEAL, EOS_STIFFENED = 0, 1
+EOS_IDEAL, EOS_STIFFENED, EOS_NOBLE_ABEL = 0, 1, 2
+EOS_KIND = {"ideal_gas": EOS_IDEAL, "stiffened_gas": EOS_STIFFENED, "noble_abel": EOS_NOBLE_ABEL,
+            "noble_abel_gas": EOS_NOBLE_ABEL, "real_gas": EOS_NOBLE_ABEL}
 LAMBDA = {"kapila": 0, "allaire": 1, "equal_velocity": 2}
 # boundary codes shared with include/dimhip.h; 3 = the name "reflection",
 # which the reference accepts but does not flip (boundary.hpp:219-221)
@@ -37,18 +39,22 @@ class Params(C.Structure):
         ("pinf", C.c_double * 4),
         ("lambda_model", C.c_int),
         ("alpha_source_floor", C.c_double),
+        ("covolume", C.c_double * 4),
     ]
 
 
 def make_params(dim, materials, lambda_model="kapila", alpha_source_floor=0.0):
-    """materials: list of (kind, gamma, p_inf) with kind 'ideal_gas'|'stiffened_gas'."""
+    """materials: list of (kind, gamma, p_inf[, covolume]) with kind 'ideal_gas',
+    'stiffened_gas' or 'noble_abel' (or its code)."""
     p = Params()
     p.dim = dim
     p.nmat = len(materials)
-    for k, (kind, g, pinf) in enumerate(materials):
-        p.kind[k] = EOS_STIFFENED if kind in ("stiffened_gas", EOS_STIFFENED) else EOS_IDEAL
+    for k, mat in enumerate(materials):
+        kind, g, pinf = mat[:3]
+        p.kind[k] = EOS_KIND[kind] if isinstance(kind, str) else int(kind)
         p.gamma[k] = g
         p.pinf[k] = pinf
+        p.covolume[k] = mat[3] if len(mat) > 3 else 0.0
     p.lambda_model = LAMBDA[lambda_model] if isinstance(lambda_model, str) else lambda_model
     p.alpha_source_floor = alpha_source_floor
     return p

@@ -35,6 +35,7 @@ struct RefParams {
     double pinf[4];
     int lambda_model;  // 0 kapila, 1 allaire, 2 equal_velocity
     double alpha_source_floor;
+    double covolume[4];  // noble-Abel b
 };
 
 dim::EOSParams make_params(const RefParams* p)
@@ -42,9 +43,12 @@ dim::EOSParams make_params(const RefParams* p)
     dim::EOSParams out;
     out.material.resize(p->nmat);
     for (int k = 0; k < p->nmat; ++k) {
-        out.material[k].kind = (p->kind[k] == 1) ? EOSKind::stiffened_gas : EOSKind::ideal_gas;
+        out.material[k].kind = (p->kind[k] == 2)   ? EOSKind::noble_abel
+                               : (p->kind[k] == 1) ? EOSKind::stiffened_gas
+                                                   : EOSKind::ideal_gas;
         out.material[k].gamma = p->gamma[k];
         out.material[k].p_inf = p->pinf[k];
+        out.material[k].covolume = p->covolume[k];
     }
     return out;
 }

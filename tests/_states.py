@@ -1,11 +1,16 @@
 """Seeded state generators shared by the tests (inputs only)."""
 import numpy as np
 
+from diffuse_vs_gfm_mms_amd.workloads import material_energy_density
+
 WATER = ("stiffened_gas", 4.4, 6.0e8)
 AIR = ("ideal_gas", 1.4, 0.0)
 HELIUM = ("ideal_gas", 1.648, 0.0)
+NA_AIR = ("noble_abel", 1.4, 0.0, 1.0e-3)
 MATSETS = {"water_air": [WATER, AIR], "air_helium": [AIR, HELIUM],
-           "air": [AIR], "water_air_helium": [WATER, AIR, HELIUM]}
+           "air": [AIR], "water_air_helium": [WATER, AIR, HELIUM],
+           "water_na_air": [WATER, NA_AIR], "water_na_air_helium": [WATER, NA_AIR, HELIUM],
+           "na_air": [NA_AIR]}
 
 
 def random_states(materials, dim, n, rng, extreme=True):
@@ -31,8 +36,8 @@ def random_states(materials, dim, n, rng, extreme=True):
     m[choice == 6, 0] = -1e-9
     rt = m.sum(1)
     e = np.zeros(n)
-    for k, (kind, g, pinf) in enumerate(materials):
-        e += a[:, k] * (p + (g * pinf if kind == "stiffened_gas" else 0.0)) / (g - 1.0)
+    for k, mat in enumerate(materials):
+        e += material_energy_density(mat, a[:, k], rho[:, k], p)
     E = e + 0.5 * rt * (u ** 2).sum(1)
     sel = choice == 7
     E[sel] *= rng.uniform(-0.5, 0.5, size=sel.sum())

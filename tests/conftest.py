@@ -22,4 +22,4 @@ def oracle():
 def golden():
     import numpy as np
     d = os.path.join(ROOT, "tests", "golden")
-    return {name: np.load(os.path.join(d, name + ".npz")) for name in ("functions", "lines", "steps")}
+    return {name: np.load(os.path.join(d, name + ".npz")) for name in ("functions", "lines", "steps", "noble_abel")}

@@ -95,12 +95,60 @@ def steps(ref):
     np.savez_compressed(os.path.join(OUT, "steps.npz"), **data)
 
 
+def noble_abel(ref, rng):
+    """Noble-Abel material sets (water + noble-Abel air, and with helium as a
+    third material): functions, lines and a few steps of a coarse Gorsse TC9
+    water/air bubble collapse and of a 1D water/noble-Abel-air tube."""
+    data = {}
+    for dim in (1, 2, 3):
+        for ms in ("water_na_air", "water_na_air_helium", "na_air"):
+            for lm in ("kapila", "allaire", "equal_velocity"):
+                mats = MATSETS[ms]
+                p = make_params(dim, mats, lm)
+                key = f"d{dim}_{ms}_{lm}"
+                U = random_states(mats, dim, 300, rng)
+                V = random_states(mats, dim, 300, rng)
+                P = ref.cons_to_prim(p, U)
+                data[key + "_U"] = U
+                data[key + "_V"] = V
+                data[key + "_c2p"] = P
+                data[key + "_p2c"] = ref.prim_to_cons(p, P)
+                data[key + "_repair"] = ref.repair_state(p, U)
+                data[key + "_rhs"] = ref.alpha_rhs(p, P)
+                for d in range(dim):
+                    F, fv = ref.hllc(p, d, U, V)
+                    data[key + f"_hllc{d}_F"] = F
+                    data[key + f"_hllc{d}_fv"] = fv
+        mats = MATSETS["water_na_air"]
+        p = make_params(dim, mats)
+        for n in (3, 16, 64):
+            for d in range(dim):
+                key = f"line_d{dim}_n{n}_dir{d}"
+                U = random_line(mats, dim, n, rng)
+                data[key + "_U"] = U
+                data[key + "_out"] = ref.solve_line(p, d, U, 1e-3, 2e-8)
+    c = W.gorsse_tc9_2d(48, 40)
+    p = make_params(2, c.materials)
+    data["gorsse_U0"] = c.U.copy()
+    U, dts, _ = ref.advance(p, c.N, c.dx, c.bc_lo, c.bc_hi, c.U, c.cfl, 1.0, nsteps=5)
+    data["gorsse_U5"] = U
+    data["gorsse_dt"] = dts
+    c = W.water_na_air_tube_1d(200)
+    p = make_params(1, c.materials)
+    data["tube_U0"] = c.U.copy()
+    U, dts, _ = ref.advance(p, c.N, c.dx, c.bc_lo, c.bc_hi, c.U, c.cfl, 1.0, nsteps=20)
+    data["tube_U20"] = U
+    data["tube_dt"] = dts
+    np.savez_compressed(os.path.join(OUT, "noble_abel.npz"), **data)
+
+
 def main():
     ref = Checker("ref")
     rng = np.random.default_rng(20261203)
     functions(ref, rng)
     lines(ref, rng)
     steps(ref)
+    noble_abel(ref, np.random.default_rng(20261204))
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))

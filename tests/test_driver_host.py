@@ -28,6 +28,8 @@ CASE_FILES = {
     "coated_bubble_2d.txt": 2,
     "box_regions_2d.txt": 2,
     "bubble_collapse_3d.txt": 3,
+    "gorsse_tc9_2d.txt": 2,
+    "three_material_1d.txt": 1,
 }
 
 
@@ -38,7 +40,13 @@ def materials_of(path):
         if line.startswith("material ") or line.startswith("material="):
             parts = [t.strip() for t in line.split("=", 1)[1].split(",")]
             kv = dict(t.split("=") for t in parts[2:])
-            mats[int(parts[0])] = (parts[1], float(kv["gamma"]), float(kv.get("p_inf", 0.0)))
+            kind = parts[1]
+            # p_inf is read for stiffened gas only, the co-volume (key b, else
+            # covolume) for noble-Abel only (material_builder.hpp:21-38)
+            pinf = float(kv.get("p_inf", 0.0)) if kind == "stiffened_gas" else 0.0
+            b = float(kv.get("b", kv.get("covolume", 0.0))) if kind in ("noble_abel", "noble_abel_gas",
+                                                                         "real_gas") else 0.0
+            mats[int(parts[0])] = (kind, float(kv["gamma"]), pinf, b)
     return [mats[k] for k in sorted(mats)]
 
 

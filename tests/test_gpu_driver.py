@@ -34,6 +34,8 @@ CASE_FILES = {
     "coated_bubble_2d.txt": 2,
     "box_regions_2d.txt": 2,
     "bubble_collapse_3d.txt": 3,
+    "gorsse_tc9_2d.txt": 2,
+    "three_material_1d.txt": 1,
 }
 
 

@@ -247,7 +247,8 @@ def test_helium_2048x1024_step_matches_reference():
     assert bits_equal(out, U), mismatch_report(out, U)
 
 
-@pytest.mark.parametrize("ms", ["water_air", "air_helium", "water_air_helium", "air"])
+@pytest.mark.parametrize("ms", ["water_air", "air_helium", "water_air_helium", "air", "water_na_air",
+                                "water_na_air_helium", "na_air"])
 def test_pressure_bisection_matches_oracle(oracle, ms):
     """The device pressure bisection equals the oracle's, bit for bit, over
     wide-range and adversarial inputs: tiny/huge/negative energies, vanishing
