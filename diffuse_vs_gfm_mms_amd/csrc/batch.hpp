@@ -53,7 +53,8 @@ batch_sweep_kernel(const BatchInst<Pol>* __restrict__ insts, const Work* __restr
     for (int w = beg; w < end; w += stride) {
         const Work it = work[w];
         const BatchInst<Pol>& I = insts[it.inst];
-        sweep_tile<Pol, 2, 2, DIR, L, W, 0, FUSE>(I.q[cur], I.q[cur ^ 1], I.g, I.P, I.ctl, I.cfl, it.a, it.b, 0);
+        sweep_tile<Pol, 2, 2, DIR, L, W, 0, FUSE>(I.q[cur], I.q[cur ^ 1], I.g, I.P, I.ctl, I.cfl, nullptr, nullptr,
+                                                   it.a, it.b, 0);
     }
 }
 
@@ -84,7 +85,8 @@ batch_cfl_kernel(const BatchInst<Pol>* __restrict__ insts, const Work* __restric
         for (long long c = it.a + (long long)threadIdx.x; c < it.b; c += blockDim.x) {
             typename Pol::template Cell<2, 2> U;
             Pol::template load<2, 2>(I.q[cur], I.g.pitch, c, U);
-            cand = cfl_combine<Pol>(cand, Pol::template cfl_value<2, 2>(I.P, U, I.cfl));
+            double pc;
+            cand = cfl_combine<Pol>(cand, Pol::template cfl_value<2, 2>(I.P, U, I.cfl, pc));
         }
         cfl_block_atomic<Pol, 256>(cand, I.ctl, I.ctl->slot, w);
     }

@@ -46,6 +46,11 @@ struct dimhip_ctx {
     dimhip::StepCtl* probe = nullptr;     // pinned: step control read back while steps are in flight
     hipEvent_t probe_event = nullptr;
     double* staging = nullptr;
+    // FP64 form, dim >= 2: the recovered pressure of every cell of the
+    // current state, written by the CFL pass (the fused epilogue of a step's
+    // last sweep plus the boundary copies, or the stand-alone CFL kernel) and
+    // read by the next step's first sweep; valid whenever dt_ready is
+    double* pstore = nullptr;
     long long staging_cells = 0;
     double* partial = nullptr;
     double* totals = nullptr;
@@ -84,9 +89,12 @@ namespace dimhip {
 // (dim, nmat) (dimhip.hip), dispatching on the context's precision form.
 struct Ops {
     virtual ~Ops() = default;
-    virtual void sweep(dimhip_ctx* c, int dir, int mode, bool fuse, const void* in, void* out) const = 0;
-    virtual void bc(dimhip_ctx* c, void* q, int end_step) const = 0;
-    virtual void cfl(dimhip_ctx* c, const void* q, bool unsplit, double cfl) const = 0;
+    // pin / pout / pstore: the recovered-pressure plane of the FP64 form
+    // (dimhip_ctx::pstore) or nullptr
+    virtual void sweep(dimhip_ctx* c, int dir, int mode, bool fuse, const void* in, void* out, const double* pin,
+                       double* pout) const = 0;
+    virtual void bc(dimhip_ctx* c, void* q, int end_step, double* pstore) const = 0;
+    virtual void cfl(dimhip_ctx* c, const void* q, bool unsplit, double cfl, double* pout) const = 0;
     virtual void totals(dimhip_ctx* c, const void* q) const = 0;
     virtual void pack(dimhip_ctx* c, bool in, double* staging, void* q, long long first, long long count) const = 0;
     virtual int kat(const EosParams& P, int which, int dir, int count, const double* in0, const double* in1,
@@ -127,9 +135,10 @@ inline int grid_blocks(long long work, int per)
 template<typename Pol, int D, int M>
 struct PolOps {
     using Eos = typename Pol::Eos;
-    static void sweep(dimhip_ctx* c, int dir, int mode, bool fuse, const void* in, void* out, const Eos& P);
-    static void bc(dimhip_ctx* c, void* q, int end_step, const Eos& P);
-    static void cfl(dimhip_ctx* c, const void* q, bool unsplit, double cfl, const Eos& P);
+    static void sweep(dimhip_ctx* c, int dir, int mode, bool fuse, const void* in, void* out, const double* pin,
+                      double* pout, const Eos& P);
+    static void bc(dimhip_ctx* c, void* q, int end_step, double* pstore, const Eos& P);
+    static void cfl(dimhip_ctx* c, const void* q, bool unsplit, double cfl, double* pout, const Eos& P);
     static void totals(dimhip_ctx* c, const void* q, const Eos& P);
     static void pack(dimhip_ctx* c, bool in, double* staging, void* q, long long first, long long count, const Eos& P);
     static void boundary_flux(dimhip_ctx* c, const void* q, double* integral, const Eos& PE);

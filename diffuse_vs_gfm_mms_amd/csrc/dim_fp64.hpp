@@ -372,9 +372,11 @@ __device__ __forceinline__ double total_density(const State<D, M>& U)
     return r;
 }
 
-// primitives.hpp:111-162
-template<int D, int M, typename EP>
-__device__ __forceinline__ void cons_to_prim(const EP& P, const State<D, M>& U, Prim<D, M>& R)
+// cons_to_prim up to the pressure recovery: fractions, velocities and
+// material densities into R; returns rho_total and sets the specific
+// internal energy ie
+template<int D, int M>
+__device__ __forceinline__ double cons_to_prim_base(const State<D, M>& U, Prim<D, M>& R, double& ie)
 {
     full_alpha<D, M>(U, R.alpha);
     sanitise_alpha<M>(R.alpha, 1e-12);
@@ -392,7 +394,16 @@ __device__ __forceinline__ void cons_to_prim(const EP& P, const State<D, M>& U, 
     for (int k = 0; k < M; ++k)
         R.rho[k] = smax(safe_div(finite_or(U.m[k]), smax(R.alpha[k], 1e-12)), 1e-12);
     const double kinetic = 0.5 * rho_total * v2;
-    const double ie = safe_div(finite_or(U.E, kinetic) - kinetic, rho_total);
+    ie = safe_div(finite_or(U.E, kinetic) - kinetic, rho_total);
+    return rho_total;
+}
+
+// primitives.hpp:111-162
+template<int D, int M, typename EP>
+__device__ __forceinline__ void cons_to_prim(const EP& P, const State<D, M>& U, Prim<D, M>& R)
+{
+    double ie;
+    const double rho_total = cons_to_prim_base<D, M>(U, R, ie);
     R.p = smax(pressure_from_internal_energy<M>(P, rho_total, ie, R.alpha, R.rho), 1e-12);
 }
 

@@ -86,25 +86,27 @@ void ev_collect(dimhip_ctx* c)
 // on whether a material is noble-Abel) to the policy launchers.
 template<int D, int M>
 struct OpsImpl final : Ops {
-    void sweep(dimhip_ctx* c, int dir, int mode, bool fuse, const void* in, void* out) const override
+    void sweep(dimhip_ctx* c, int dir, int mode, bool fuse, const void* in, void* out, const double* pin,
+               double* pout) const override
     {
-        if (c->desc.precision != DIMHIP_FP64) PolOps<Fp32Mixed, D, M>::sweep(c, dir, mode, fuse, in, out, c->eosm);
-        else if (c->eos.noble_abel) PolOps<Fp64NA, D, M>::sweep(c, dir, mode, fuse, in, out, c->eos_na);
-        else PolOps<Fp64, D, M>::sweep(c, dir, mode, fuse, in, out, c->eos);
+        if (c->desc.precision != DIMHIP_FP64)
+            PolOps<Fp32Mixed, D, M>::sweep(c, dir, mode, fuse, in, out, nullptr, nullptr, c->eosm);
+        else if (c->eos.noble_abel) PolOps<Fp64NA, D, M>::sweep(c, dir, mode, fuse, in, out, pin, pout, c->eos_na);
+        else PolOps<Fp64, D, M>::sweep(c, dir, mode, fuse, in, out, pin, pout, c->eos);
     }
 
-    void bc(dimhip_ctx* c, void* q, int end_step) const override
+    void bc(dimhip_ctx* c, void* q, int end_step, double* pstore) const override
     {
-        if (c->desc.precision != DIMHIP_FP64) PolOps<Fp32Mixed, D, M>::bc(c, q, end_step, c->eosm);
-        else if (c->eos.noble_abel) PolOps<Fp64NA, D, M>::bc(c, q, end_step, c->eos_na);
-        else PolOps<Fp64, D, M>::bc(c, q, end_step, c->eos);
+        if (c->desc.precision != DIMHIP_FP64) PolOps<Fp32Mixed, D, M>::bc(c, q, end_step, nullptr, c->eosm);
+        else if (c->eos.noble_abel) PolOps<Fp64NA, D, M>::bc(c, q, end_step, pstore, c->eos_na);
+        else PolOps<Fp64, D, M>::bc(c, q, end_step, pstore, c->eos);
     }
 
-    void cfl(dimhip_ctx* c, const void* q, bool unsplit, double cfl) const override
+    void cfl(dimhip_ctx* c, const void* q, bool unsplit, double cfl, double* pout) const override
     {
-        if (c->desc.precision != DIMHIP_FP64) PolOps<Fp32Mixed, D, M>::cfl(c, q, unsplit, cfl, c->eosm);
-        else if (c->eos.noble_abel) PolOps<Fp64NA, D, M>::cfl(c, q, unsplit, cfl, c->eos_na);
-        else PolOps<Fp64, D, M>::cfl(c, q, unsplit, cfl, c->eos);
+        if (c->desc.precision != DIMHIP_FP64) PolOps<Fp32Mixed, D, M>::cfl(c, q, unsplit, cfl, nullptr, c->eosm);
+        else if (c->eos.noble_abel) PolOps<Fp64NA, D, M>::cfl(c, q, unsplit, cfl, pout, c->eos_na);
+        else PolOps<Fp64, D, M>::cfl(c, q, unsplit, cfl, pout, c->eos);
     }
 
     // the FP64 totals and packing touch no material law
@@ -284,10 +286,15 @@ void enqueue_step(dimhip_ctx* c, double cfl, double dt_max_host)
 {
     const bool unsplit = c->desc.time_update == DIMHIP_UNSPLIT;
     const int D = c->desc.dim;
+    // The CFL pass of the step's input state (the previous step's fused
+    // epilogue, or the stand-alone kernel below) leaves its recovered
+    // pressures in pstore (FP64, dim >= 2): the sweeps that read that state
+    // take them.
+    double* ps = c->pstore;
     if (!c->dt_ready || c->dt_ready_cfl != cfl) {
         const int e = ev_begin(c);
         reset_dtmin_kernel<<<1, kDtSlots, 0, c->stream>>>(c->ctl, 0);
-        c->ops->cfl(c, state_ptr(c, c->cur), unsplit, cfl);
+        c->ops->cfl(c, state_ptr(c, c->cur), unsplit, cfl, ps);
         ev_end(c, e, KC_CFL);
     }
     begin_step_kernel<<<1, kDtSlots, 0, c->stream>>>(c->ctl, dt_max_host);
@@ -297,11 +304,12 @@ void enqueue_step(dimhip_ctx* c, double cfl, double dt_max_host)
         for (int dir = 0; dir < D; ++dir) {
             void* in = state_ptr(c, c->cur);
             void* out = state_ptr(c, c->cur ^ 1);
+            const bool last = dir == D - 1;
             int e = ev_begin(c);
-            c->ops->sweep(c, dir, 0, fuse && dir == D - 1, in, out);
+            c->ops->sweep(c, dir, 0, fuse && last, in, out, dir == 0 ? ps : nullptr, fuse && last ? ps : nullptr);
             ev_end(c, e, KC_SWEEP);
             e = ev_begin(c);
-            c->ops->bc(c, out, dir == D - 1);
+            c->ops->bc(c, out, last, fuse && last ? ps : nullptr);
             ev_end(c, e, KC_BC);
             c->cur ^= 1;
         }
@@ -311,11 +319,11 @@ void enqueue_step(dimhip_ctx* c, double cfl, double dt_max_host)
         copy_state(c, base, acc);
         for (int dir = 0; dir < D; ++dir) {
             const int e = ev_begin(c);
-            c->ops->sweep(c, dir, 1, false, base, acc);
+            c->ops->sweep(c, dir, 1, false, base, acc, ps, nullptr);
             ev_end(c, e, KC_SWEEP);
         }
         const int e = ev_begin(c);
-        c->ops->bc(c, acc, 1);
+        c->ops->bc(c, acc, 1, nullptr);
         ev_end(c, e, KC_BC);
         c->cur ^= 1;
     }
@@ -450,6 +458,10 @@ int dimhip_create(const dimhip_desc* desc, dimhip_ctx** out)
     HIP_OK(nullptr, hipMalloc(&c->q[1], bytes));
     HIP_OK(nullptr, hipMemsetAsync(c->q[0], 0, bytes, c->stream));
     HIP_OK(nullptr, hipMemsetAsync(c->q[1], 0, bytes, c->stream));
+    // the recovered-pressure plane: one double per cell (not in 1D, where the
+    // step's only sweep would read and write it)
+    if (desc->precision == DIMHIP_FP64 && desc->dim >= 2)
+        HIP_OK(nullptr, hipMalloc(&c->pstore, sizeof(double) * (size_t)g.pitch));
     HIP_OK(nullptr, hipMalloc(&c->ctl, kCtlAllocBytes));
     HIP_OK(nullptr, hipHostMalloc(&c->ctl_host, sizeof(StepCtl), hipHostMallocDefault));
     HIP_OK(nullptr, hipHostMalloc(&c->probe, sizeof(StepCtl), hipHostMallocDefault));
@@ -489,6 +501,7 @@ int dimhip_destroy(dimhip_ctx* c)
     for (auto e : c->ev_pool) hipEventDestroy(e);
     hipFree(c->q[0]);
     hipFree(c->q[1]);
+    hipFree(c->pstore);
     hipFree(c->ctl);
     hipHostFree(c->ctl_host);
     hipHostFree(c->probe);
@@ -644,7 +657,7 @@ int dimhip_compute_dt(dimhip_ctx* c, double cfl, double dt_max, double* dt_out)
     if (dt_max <= 0.0) return fail(c, "dim::compute_dt_cfl: dt_max must be positive");
     HIP_OK(c, hipSetDevice(c->desc.device));
     reset_dtmin_kernel<<<1, kDtSlots, 0, c->stream>>>(c->ctl, 0);
-    c->ops->cfl(c, state_ptr(c, c->cur), c->desc.time_update == DIMHIP_UNSPLIT, cfl);
+    c->ops->cfl(c, state_ptr(c, c->cur), c->desc.time_update == DIMHIP_UNSPLIT, cfl, c->pstore);
     reduce_dtmin_kernel<<<1, kDtSlots, 0, c->stream>>>(c->ctl);
     HIP_OK(c, hipGetLastError());
     if (fetch_ctl(c)) return 1;
@@ -661,7 +674,7 @@ int dimhip_sweep(dimhip_ctx* c, int dir, double dt)
     if (dir < 0 || dir >= c->desc.dim) return fail(c, "dimhip_sweep: direction out of range");
     HIP_OK(c, hipSetDevice(c->desc.device));
     set_active_kernel<<<1, 1, 0, c->stream>>>(c->ctl, dt);
-    c->ops->sweep(c, dir, 0, false, state_ptr(c, c->cur), state_ptr(c, c->cur ^ 1));
+    c->ops->sweep(c, dir, 0, false, state_ptr(c, c->cur), state_ptr(c, c->cur ^ 1), nullptr, nullptr);
     HIP_OK(c, hipGetLastError());
     c->cur ^= 1;
     c->dt_ready = false;
@@ -674,7 +687,7 @@ int dimhip_apply_bc(dimhip_ctx* c)
     if (!c) return fail(c, "dimhip_apply_bc: null context");
     HIP_OK(c, hipSetDevice(c->desc.device));
     set_active_kernel<<<1, 1, 0, c->stream>>>(c->ctl, 0.0);
-    c->ops->bc(c, state_ptr(c, c->cur), 0);
+    c->ops->bc(c, state_ptr(c, c->cur), 0, nullptr);
     HIP_OK(c, hipGetLastError());
     c->dt_ready = false;
     return fetch_ctl(c);

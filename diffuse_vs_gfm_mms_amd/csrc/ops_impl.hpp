@@ -9,55 +9,60 @@
 namespace dimhip {
 
 template<typename Pol, int D, int M, int DIR, int MODE, bool FUSE>
-void launch_sweep(dimhip_ctx* c, const void* in, void* out, const typename Pol::Eos& P, const CflArgs& ca)
+void launch_sweep(dimhip_ctx* c, const void* in, void* out, const double* pin, double* pout, const typename Pol::Eos& P,
+                  const CflArgs& ca)
 {
     const Grid& g = c->grid;
     if constexpr (DIR == 0) {
         constexpr int T = kSweepL0 - 6;
         dim3 blk(kSweepL0, kSweepW0, 1);
         dim3 grd((g.N[0] + T - 1) / T, (g.N[1] + kSweepW0 - 1) / kSweepW0, g.N[2]);
-        sweep_kernel<Pol, D, M, 0, kSweepL0, kSweepW0, MODE, FUSE><<<grd, blk, 0, c->stream>>>(in, out, g, P, c->ctl, ca);
+        sweep_kernel<Pol, D, M, 0, kSweepL0, kSweepW0, MODE, FUSE><<<grd, blk, 0, c->stream>>>(in, out, g, P, c->ctl, ca, pin,
+                                                                                                  pout);
     } else {
         constexpr int T = kSweepL1 - 6;
         dim3 blk(kSweepW1, kSweepL1, 1);
         dim3 grd((g.N[0] + kSweepW1 - 1) / kSweepW1, (g.N[DIR] + T - 1) / T, DIR == 1 ? g.N[2] : g.N[1]);
-        sweep_kernel<Pol, D, M, DIR, kSweepL1, kSweepW1, MODE, FUSE><<<grd, blk, 0, c->stream>>>(in, out, g, P, c->ctl, ca);
+        sweep_kernel<Pol, D, M, DIR, kSweepL1, kSweepW1, MODE, FUSE><<<grd, blk, 0, c->stream>>>(in, out, g, P, c->ctl, ca, pin,
+                                                                                                    pout);
     }
 }
 
 template<typename Pol, int D, int M, int DIR>
-void sweep_modes(dimhip_ctx* c, int mode, bool fuse, const void* in, void* out, const typename Pol::Eos& P)
+void sweep_modes(dimhip_ctx* c, int mode, bool fuse, const void* in, void* out, const double* pin, double* pout,
+                 const typename Pol::Eos& P)
 {
     const CflArgs ca = make_cfl_args(c, c->dt_ready_cfl, c->desc.time_update == DIMHIP_UNSPLIT);
-    if (mode == 0 && !fuse) launch_sweep<Pol, D, M, DIR, 0, false>(c, in, out, P, ca);
-    else if (mode == 0 && fuse) launch_sweep<Pol, D, M, DIR, 0, true>(c, in, out, P, ca);
-    else launch_sweep<Pol, D, M, DIR, 1, false>(c, in, out, P, ca);
+    if (mode == 0 && !fuse) launch_sweep<Pol, D, M, DIR, 0, false>(c, in, out, pin, nullptr, P, ca);
+    else if (mode == 0 && fuse) launch_sweep<Pol, D, M, DIR, 0, true>(c, in, out, pin, pout, P, ca);
+    else launch_sweep<Pol, D, M, DIR, 1, false>(c, in, out, pin, nullptr, P, ca);
 }
 
 template<typename Pol, int D, int M>
-void PolOps<Pol, D, M>::sweep(dimhip_ctx* c, int dir, int mode, bool fuse, const void* in, void* out, const Eos& P)
+void PolOps<Pol, D, M>::sweep(dimhip_ctx* c, int dir, int mode, bool fuse, const void* in, void* out,
+                              const double* pin, double* pout, const Eos& P)
 {
-    if (dir == 0) sweep_modes<Pol, D, M, 0>(c, mode, fuse, in, out, P);
+    if (dir == 0) sweep_modes<Pol, D, M, 0>(c, mode, fuse, in, out, pin, pout, P);
     if constexpr (D >= 2)
-        if (dir == 1) sweep_modes<Pol, D, M, 1>(c, mode, fuse, in, out, P);
+        if (dir == 1) sweep_modes<Pol, D, M, 1>(c, mode, fuse, in, out, pin, pout, P);
     if constexpr (D >= 3)
-        if (dir == 2) sweep_modes<Pol, D, M, 2>(c, mode, fuse, in, out, P);
+        if (dir == 2) sweep_modes<Pol, D, M, 2>(c, mode, fuse, in, out, pin, pout, P);
 }
 
 template<typename Pol, int D, int M>
-void PolOps<Pol, D, M>::bc(dimhip_ctx* c, void* q, int end_step, const Eos& P)
+void PolOps<Pol, D, M>::bc(dimhip_ctx* c, void* q, int end_step, double* pstore, const Eos& P)
 {
     const long long total = c->bc.face_start[2 * D];
     const int blocks = grid_blocks(total, 256);
-    bc_kernel<Pol, D, M><<<blocks, 256, 0, c->stream>>>(q, c->grid, P, c->bc, c->ctl, end_step);
+    bc_kernel<Pol, D, M><<<blocks, 256, 0, c->stream>>>(q, c->grid, P, c->bc, c->ctl, end_step, pstore);
 }
 
 template<typename Pol, int D, int M>
-void PolOps<Pol, D, M>::cfl(dimhip_ctx* c, const void* q, bool unsplit, double cfl, const Eos& P)
+void PolOps<Pol, D, M>::cfl(dimhip_ctx* c, const void* q, bool unsplit, double cfl, double* pout, const Eos& P)
 {
     const CflArgs ca = make_cfl_args(c, cfl, unsplit);
     const int blocks = grid_blocks(c->grid.ncells, 256 * 4);
-    cfl_kernel<Pol, D, M><<<blocks, 256, 0, c->stream>>>(q, c->grid, P, c->ctl, ca);
+    cfl_kernel<Pol, D, M><<<blocks, 256, 0, c->stream>>>(q, c->grid, P, c->ctl, ca, pout);
 }
 
 template<typename Pol, int D, int M>

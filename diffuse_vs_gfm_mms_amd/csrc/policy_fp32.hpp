@@ -747,6 +747,8 @@ struct Fp32Mixed {
     // combined by max in FP64; the block's dt is 1/rate (reciprocal is monotone,
     // so this equals the minimum of the per-cell 1/rate)
     static constexpr bool kCflRate = true;
+    // the recovered pressure is not carried between sweeps (closed form)
+    static constexpr bool kStoredPressure = false;
     using R = float;
     using Eos = EosMixed;
     template<int D, int M> using St = f32::St<D, M>;
@@ -1040,7 +1042,8 @@ struct Fp32Mixed {
     // (|u_d|+c)/(cfl dx_d); unsplit: sum_d (|u_d|+c)/dx_d / cfl), from the
     // FP32 wave speed; 0 when no direction has |u_d|+c > 1e-14
     template<int D, int M>
-    static __device__ __forceinline__ double cfl_value(const Eos& P, const Cell<D, M>& C, const CflArgs& a)
+    static __device__ __forceinline__ double cfl_value(const Eos& P, const Cell<D, M>& C, const CflArgs& a,
+                                                       double& /*p_out: unused*/)
     {
         St<D, M> U;
         to_state<D, M>(C, U);

@@ -35,6 +35,10 @@ struct Fp64T {
     static constexpr bool kReloadInput = true;
     // CFL reduction: each cell contributes its dt candidate, combined by min
     static constexpr bool kCflRate = false;
+    // the CFL pass stores each cell's recovered pressure (cons_to_prim's p,
+    // bisection included) and the next sweep's first phase reads it back
+    // instead of repeating the bisection: same bits, one bisection less
+    static constexpr bool kStoredPressure = true;
     using R = double;
     using Eos = std::conditional_t<NA, EosParamsNA, EosParams>;
     template<int D, int M> using St = State<D, M>;
@@ -100,6 +104,15 @@ struct Fp64T {
 
     template<int D, int M>
     static __device__ __forceinline__ void to_state(const Cell<D, M>& c, St<D, M>& U) { U = c; }
+
+    // cons_to_prim with the pressure recovered earlier for the same state
+    template<int D, int M>
+    static __device__ __forceinline__ void cons_to_prim_stored(const Eos&, const St<D, M>& U, double p, Pr<D, M>& Q)
+    {
+        double ie;
+        cons_to_prim_base<D, M>(U, Q, ie);
+        Q.p = p;
+    }
 
     template<int D, int M>
     static __device__ __forceinline__ void cons_to_prim(const Eos& P, const St<D, M>& U, Pr<D, M>& Q)
@@ -219,12 +232,15 @@ struct Fp64T {
     template<int D, int M>
     static __device__ __forceinline__ void repair_cell(const Eos& P, Cell<D, M>& U) { repair_state<D, M>(P, U); }
 
-    // CFL candidate (cfl.hpp:46-63 / 94-116)
+    // CFL candidate (cfl.hpp:46-63 / 94-116); p_out receives the cell's
+    // recovered pressure, which the next sweep reuses (kStoredPressure)
     template<int D, int M>
-    static __device__ __forceinline__ double cfl_value(const Eos& P, const Cell<D, M>& U, const CflArgs& a)
+    static __device__ __forceinline__ double cfl_value(const Eos& P, const Cell<D, M>& U, const CflArgs& a,
+                                                       double& p_out)
     {
         Prim<D, M> Q;
         dimhip::cons_to_prim<D, M>(P, U, Q);
+        p_out = Q.p;
         const double c = mixture_sound_speed<M>(P, total_density<D, M>(U), Q.p, Q.alpha, Q.rho);
         double best = bits_to_double(kInfBits);
         if (!a.unsplit) {

@@ -181,11 +181,16 @@ __device__ __forceinline__ void xcd_block(int& bx, int& by, int& bz)
 // FUSE_CFL: reduce next step's CFL candidate over interior cells too (ring
 // cells become copies of interior cells with |u| preserved in the boundary
 // pass, so the interior minimum is the next step's dt).
+// pin (Pol::kStoredPressure): the recovered pressure of every input cell,
+// stored by the CFL pass that read the same state; phase 1 takes it instead
+// of repeating the bisection.  pout (with FUSE_CFL): the recovered pressure
+// of every interior output cell, for the next sweep.
 // ---------------------------------------------------------------------------
 template<typename Pol, int D, int M, int DIR, int L, int W, int MODE, bool FUSE_CFL>
 __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* __restrict__ qout, const Grid& g,
                                            const typename Pol::Eos& P, StepCtl* __restrict__ ctl,
-                                           const CflArgs& cflargs, int bx, int by, int bz)
+                                           const CflArgs& cflargs, const double* __restrict__ pin,
+                                           double* __restrict__ pout, int bx, int by, int bz)
 {
     using R = typename Pol::R;
     using St = typename Pol::template St<D, M>;
@@ -259,7 +264,12 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     if (in) {
         Pol::template load<D, M>(qin, g.pitch, cell, Uc);
         Pol::template to_state<D, M>(Uc, U);
-        Pol::template cons_to_prim<D, M>(P, U, Pn);
+        if constexpr (Pol::kStoredPressure) {
+            if (pin) Pol::template cons_to_prim_stored<D, M>(P, U, pin[cell], Pn);
+            else Pol::template cons_to_prim<D, M>(P, U, Pn);
+        } else {
+            Pol::template cons_to_prim<D, M>(P, U, Pn);
+        }
         Pl = Pn;
         Pol::template enforce_c2p<D, M>(Pl);
         Pol::template alpha_rhs<D, M>(P, Pn, rhs_n);
@@ -414,7 +424,12 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
 #pragma unroll
             for (int d = 0; d < D; ++d)
                 if (g.N[d] > 1 && (idx[d] == 0 || idx[d] == g.N[d] - 1)) interior = false;
-            if (interior) cand = Pol::template cfl_value<D, M>(P, r, cflargs);
+            if (interior) {
+                double pc = 0.0;
+                cand = Pol::template cfl_value<D, M>(P, r, cflargs, pc);
+                if constexpr (Pol::kStoredPressure)
+                    if (pout) pout[cell] = pc;
+            }
         }
     }
     if (FUSE_CFL)
@@ -429,11 +444,11 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
 template<typename Pol, int D, int M, int DIR, int L, int W, int MODE, bool FUSE_CFL>
 __global__ void __launch_bounds__(L* W) __attribute__((amdgpu_waves_per_eu(Pol::template min_waves_per_eu<D, M>())))
 sweep_kernel(const void* __restrict__ qin, void* __restrict__ qout, Grid g, typename Pol::Eos P,
-             StepCtl* __restrict__ ctl, CflArgs cflargs)
+             StepCtl* __restrict__ ctl, CflArgs cflargs, const double* __restrict__ pin, double* __restrict__ pout)
 {
     int bx, by, bz;
     xcd_block(bx, by, bz);
-    sweep_tile<Pol, D, M, DIR, L, W, MODE, FUSE_CFL>(qin, qout, g, P, ctl, cflargs, bx, by, bz);
+    sweep_tile<Pol, D, M, DIR, L, W, MODE, FUSE_CFL>(qin, qout, g, P, ctl, cflargs, pin, pout, bx, by, bz);
 }
 
 // ---------------------------------------------------------------------------
@@ -514,7 +529,7 @@ __device__ __forceinline__ void nonreflective_state(const typename Pol::Eos& P, 
 // boundary treatment of ring cell number t (0 <= t < bc.face_start[2D])
 template<typename Pol, int D, int M>
 __device__ __forceinline__ void bc_cell(void* __restrict__ q, const Grid& g, const typename Pol::Eos& P,
-                                        const BcArgs& bc, long long t)
+                                        const BcArgs& bc, long long t, double* __restrict__ pstore = nullptr)
 {
     using Cell = typename Pol::template Cell<D, M>;
     {
@@ -575,6 +590,9 @@ __device__ __forceinline__ void bc_cell(void* __restrict__ q, const Grid& g, con
             for (int d = 0; d < D; ++d)
                 if ((idx[d] == 0 && bc.bc_lo[d] == BC_REFLECTIVE) || (idx[d] == g.N[d] - 1 && bc.bc_hi[d] == BC_REFLECTIVE))
                     st.mom[d] *= -1;
+            // a copy, with or without the normal momentum flipped, keeps the
+            // source's recovered pressure bit for bit
+            if (pstore) pstore[lin] = pstore[sl];
         }
         Pol::template store<D, M>(q, g.pitch, lin, st);
     }
@@ -589,20 +607,22 @@ __device__ __forceinline__ void end_step_ctl(StepCtl* ctl)
 
 template<typename Pol, int D, int M>
 __global__ void __launch_bounds__(256)
-bc_kernel(void* __restrict__ q, Grid g, typename Pol::Eos P, BcArgs bc, StepCtl* __restrict__ ctl, int end_step)
+bc_kernel(void* __restrict__ q, Grid g, typename Pol::Eos P, BcArgs bc, StepCtl* __restrict__ ctl, int end_step,
+          double* __restrict__ pstore)
 {
     const long long total = bc.face_start[2 * D];
     const bool active = ctl->active;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; active && t < total;
          t += (long long)gridDim.x * blockDim.x)
-        bc_cell<Pol, D, M>(q, g, P, bc, t);
+        bc_cell<Pol, D, M>(q, g, P, bc, t, pstore);
     if (end_step && blockIdx.x == 0 && threadIdx.x == 0 && active) end_step_ctl(ctl);
 }
 
 // compute_dt_cfl over every cell into ctl->dtmin[ctl->slot]
 template<typename Pol, int D, int M>
 __global__ void __launch_bounds__(256)
-cfl_kernel(const void* __restrict__ q, Grid g, typename Pol::Eos P, StepCtl* __restrict__ ctl, CflArgs a)
+cfl_kernel(const void* __restrict__ q, Grid g, typename Pol::Eos P, StepCtl* __restrict__ ctl, CflArgs a,
+           double* __restrict__ pout)
 {
     using Cell = typename Pol::template Cell<D, M>;
     double cand = cfl_identity<Pol>();
@@ -610,7 +630,10 @@ cfl_kernel(const void* __restrict__ q, Grid g, typename Pol::Eos P, StepCtl* __r
          c += (long long)gridDim.x * blockDim.x) {
         Cell U;
         Pol::template load<D, M>(q, g.pitch, c, U);
-        cand = cfl_combine<Pol>(cand, Pol::template cfl_value<D, M>(P, U, a));
+        double pc = 0.0;
+        cand = cfl_combine<Pol>(cand, Pol::template cfl_value<D, M>(P, U, a, pc));
+        if constexpr (Pol::kStoredPressure)
+            if (pout) pout[c] = pc;
     }
     cfl_block_atomic<Pol, 256>(cand, ctl, ctl->slot, (int)blockIdx.x);
 }
