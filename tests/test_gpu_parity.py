@@ -201,6 +201,35 @@ def test_device_loop_stops_at_t_end(oracle):
     assert bits_equal(out, U)
 
 
+@pytest.mark.parametrize("case", ["helium_2d", "collapse_3d"])
+def test_advance_through_output_times_matches_oracle(oracle, case):
+    """run_dim_case's loop over two output times (run_dim.hpp:88-111): each
+    advance call overshoots its step count, so the steps past the output
+    time run as pass-through steps on the device (an odd number of sweeps in
+    3D moves the state to the other buffer); the next call continues from
+    there with the graph.  Every state bit for bit against the oracle stepped
+    with dt_max = t_out - t."""
+    c = W.helium_shock_bubble_2d(48, 24) if case == "helium_2d" else W.bubble_collapse_3d(12)
+    p = make_params(c.dim, c.materials)
+    U = c.U.copy()
+    t = 0.0
+    outs = []
+    for k, t_out in enumerate((3.0e-5, 6.5e-5) if case == "helium_2d" else (1.0e-4, 2.3e-4)):
+        n = 0
+        while t < t_out - 1e-14:
+            U, dts, _ = oracle.advance(p, c.N, c.dx, c.bc_lo, c.bc_hi, U, c.cfl, t_out - t, nsteps=1)
+            t += dts[0]
+            n += 1
+        outs.append((t_out, n, t, U.copy()))
+    with dimhip.Solver.from_case(c) as s:
+        time = 0.0
+        for t_out, n, t_ref, U_ref in outs:
+            time, steps = s.advance(n + 7, c.cfl, t_out, time, use_graph=True)
+            assert steps == n and time == t_ref
+            out = s.download()
+            assert bits_equal(out, U_ref), (t_out, mismatch_report(out, U_ref))
+
+
 def test_conservation_totals():
     c = W.helium_shock_bubble_2d(300, 130)
     with dimhip.Solver.from_case(c) as s:
