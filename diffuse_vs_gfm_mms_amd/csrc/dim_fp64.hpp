@@ -222,7 +222,47 @@ __device__ __forceinline__ double bisect_pressure(const EP& P, double target, do
     if (!finite(flo) || !finite(fhi) || !(flo <= 0.0 && fhi >= 0.0)) return clamp_min(fallback);
 
     const double tol = 1e-10 * smax(1.0, target);
-    for (int n = 0; n < 80; ++n) {
+    int n = 0;
+    // Fast-forward over the halvings whose outcome is certain.  For pk =
+    // clamp_min(p) the residual is linear up to rounding: f(p) = S pk + C -
+    // target with S = sum_k coef_k s_k / den_k and C = sum_k coef_k b_k / den_k
+    // (numerator s_k pk + b_k: s = 1 or 1 - b rho_k, b = 0 or gamma p_inf).
+    // The rounded residual fmid that the loop below would compute differs from
+    // e = fma(S~, pk, C~ - target) by at most 15.4 u B, where B = Sabs pk +
+    // Cabs + |target| bounds the magnitudes involved (Sabs, Cabs: the same
+    // sums of absolute values; u = 2^-53; the three roundings of each term,
+    // M - 1 additions and the subtraction of target in the residual, and those
+    // of S~, C~ and the FMA in e).  So where |e| > tol + 32 u B the residual is
+    // finite, |fmid| > tol (no return) and fmid has the sign of e: the halving
+    // is taken from e without the M divisions.  The midpoints are the same
+    // expressions in the same order, so the result is the same bits.  Near
+    // the root (|e| <= tol + 32 u B) the exact evaluation takes over.
+    {
+        double S = 0.0, C = 0.0, Sabs = 0.0, Cabs = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const double w = c.coef[k] * c.rden[k];
+            const double sk = (NA && P.kind[k] == EOS_NOBLE_ABEL) ? c.omb[NA ? k : 0] : 1.0;
+            const double bk = (P.kind[k] == EOS_STIFFENED) ? P.gpinf[k] : 0.0;
+            S += w * sk;
+            C += w * bk;
+            Sabs += w * __builtin_fabs(sk);
+            Cabs += w * __builtin_fabs(bk);
+        }
+        const double Cmt = C - target;
+        const double B = Sabs * smax(hi, 1e-10) + Cabs + __builtin_fabs(target);  // pk <= max(hi, 1e-10) from here on
+        const double thr = tol + 3.5527136788005009e-15 * B;                      // 32 u B
+        if (finite(S) && finite(Cmt) && B < 1e290) {
+            for (; n < 80; ++n) {
+                const double mid = 0.5 * (lo + hi);
+                const double e = __builtin_fma(S, smax(mid, 1e-10), Cmt);
+                if (!(__builtin_fabs(e) > thr)) break;
+                if (e > 0.0) hi = mid;
+                else lo = mid;
+            }
+        }
+    }
+    for (; n < 80; ++n) {
         const double mid = 0.5 * (lo + hi);
         const double fmid = mix_energy<M, NA>(P, c, mid) - target;
         if (!finite(fmid)) return clamp_min(fallback);
@@ -233,10 +273,11 @@ __device__ __forceinline__ double bisect_pressure(const EP& P, double target, do
     return clamp_min(0.5 * (lo + hi));
 }
 
-// eos.hpp:77-143 — bracket doubling + bisection, run as the reference runs
-// it; only loop-invariant subexpressions are hoisted (mix_coef), which leaves
-// every rounded value unchanged.  (A replica that skips residuals far from the
-// root measured slower on gfx950: the loop is latency-bound, not residual-bound.)
+// eos.hpp:77-143 — bracket doubling + bisection with the reference's
+// midpoints, decisions and stopping test; loop-invariant subexpressions are
+// hoisted (mix_coef) and the halvings far from the root are decided from the
+// linear form of the residual (bisect_pressure), which leaves every rounded
+// value unchanged.
 template<int M, typename EP>
 __device__ double pressure_from_internal_energy(const EP& P, double rho_total, double e,
                                                 const double* alpha, const double* rho)
