@@ -51,10 +51,14 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=8,
                     help="steps of the reference CPU solver timed on the host (about 10-30 s)")
     ap.add_argument("--accuracy-steps", type=int, default=-1,
-                    help="steps of the FP32-mixed vs FP64 comparison at 4096x2048 (-1: as --fp32-steps, 0: skip)")
+                    help="steps of the FP32-mixed vs FP64 comparison at 4096x2048 (-1: as --fp32-steps, 0: skip "
+                         "every FP32-mixed vs FP64 comparison)")
     ap.add_argument("--c5-steps", type=int, default=500,
                     help="steps of the 3D 512^3 FP32-mixed bubble-collapse measurement (0: skip)")
     ap.add_argument("--c5-n", type=int, default=512)
+    ap.add_argument("--c5-accuracy-n", type=int, default=256,
+                    help="grid of the 3D FP32-mixed vs FP64 comparison (0: skip)")
+    ap.add_argument("--c5-accuracy-steps", type=int, default=100)
     ap.add_argument("--tube", action=argparse.BooleanOptionalAction, default=True,
                     help="the 1D water/air tube full run on the GPU and in the reference's CPU solver")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -127,35 +131,85 @@ def field_l1(a, b):
     return float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
 
 
-def accuracy_fp32(dimhip, workloads, grid, steps):
+def accuracy_vs_fp64(dimhip, case, steps, label):
     """The FP32-mixed form against the FP64 form (bit-identical to the
-    reference's solver) on the same initial state after the same number of
-    steps: normalised L1 of rho, p and alpha_0 from the device's FP64 CSV row
-    values, and each form's relative change of total mass."""
-    case = workloads.helium_shock_bubble_2d(*grid)
-    out = {}
+    reference's solver) at the same physical time: the FP64 run takes `steps`
+    steps to time T, the FP32-mixed run advances from the same initial state
+    to T (its own dt sequence, the last step clamped onto T, as at the
+    reference's output times).  Normalised L1 of rho, p and alpha_0 from the
+    device's FP64 CSV row values, and each form's relative change of total
+    mass.  The flow's own sensitivity is reported beside it: the FP64 run
+    repeated from the initial state with 1e-7 relative noise, compared at T
+    with the same metric."""
+    import numpy as np
+
     rows = {}
-    for prec in ("fp64", "fp32_mixed"):
-        with dimhip.Solver.from_case(case, precision=prec) as s:
+    out = {}
+    with dimhip.Solver.from_case(case, precision="fp64") as s:
+        U0 = s.download()
+        m0 = s.conservation()[case.nmat]
+        T, n64 = s.advance(steps, case.cfl, 1.0, 0.0, use_graph=True)
+        m1 = s.conservation()[case.nmat]
+        rows["fp64"] = s.output_rows()
+        out["fp64"] = {"steps": n64, "relative_mass_change": (m1 - m0) / m0}
+    noisy = U0 * (1.0 + 1e-7 * np.random.default_rng(12345).standard_normal(U0.shape))
+    for tag, prec, U in (("fp32_mixed", "fp32_mixed", U0), ("fp64_noise", "fp64", noisy)):
+        with dimhip.Solver(case.dim, case.N, case.dx, case.materials, case.bc_lo, case.bc_hi,
+                           precision=prec) as s:
+            s.upload(U)
             m0 = s.conservation()[case.nmat]
-            t, n = s.advance(steps, case.cfl, 1.0, 0.0, use_graph=True)
+            t, n = s.advance(4 * steps + 100, case.cfl, T, 0.0, use_graph=True)
             m1 = s.conservation()[case.nmat]
-            rows[prec] = s.output_rows()
-            out[prec] = {"time": t, "steps": n, "relative_mass_change": (m1 - m0) / m0}
+            if abs(t - T) > 1e-12 * T:
+                raise RuntimeError(f"{tag} run ended at t={t}, not at {T}")
+            rows[tag] = s.output_rows()
+            out[tag] = {"steps": n, "relative_mass_change": (m1 - m0) / m0}
+        del U
     D = case.dim
-    a, b = rows["fp32_mixed"], rows["fp64"]
+    b = rows["fp64"]
+
+    def l1s(a):
+        return {"rho": field_l1(a[:, 0], b[:, 0]), "p": field_l1(a[:, 1 + D], b[:, 1 + D]),
+                "alpha0": field_l1(a[:, 3 + D], b[:, 3 + D])}
+
+    e32, e_noise = l1s(rows["fp32_mixed"]), l1s(rows["fp64_noise"])
     return {
-        "workload": f"helium shock-bubble {grid[0]}x{grid[1]}, {steps} steps of each form from the same state",
-        "l1_rho": field_l1(a[:, 0], b[:, 0]),
-        "l1_p": field_l1(a[:, 1 + D], b[:, 1 + D]),
-        "l1_alpha0": field_l1(a[:, 3 + D], b[:, 3 + D]),
-        "time_fp64": out["fp64"]["time"],
-        "time_fp32_mixed": out["fp32_mixed"]["time"],
+        "workload": f"{label}, to the time of {steps} FP64 steps, both forms from the same state",
+        "time": T,
+        "steps_fp64": out["fp64"]["steps"],
+        "steps_fp32_mixed": out["fp32_mixed"]["steps"],
+        "l1_rho": e32["rho"],
+        "l1_p": e32["p"],
+        "l1_alpha0": e32["alpha0"],
+        "fp64_noise_1e-7_l1": e_noise,
         "relative_mass_change_fp64": out["fp64"]["relative_mass_change"],
         "relative_mass_change_fp32_mixed": out["fp32_mixed"]["relative_mass_change"],
         "mass_change_difference": out["fp32_mixed"]["relative_mass_change"] - out["fp64"]["relative_mass_change"],
-        "note": "open boundaries: mass changes by the boundary flux; the forms' difference is the FP32 form's "
-                "extra drift (closed-box drift: tests/test_gpu_fp32_mixed.py::test_closed_box_mass_drift)",
+    }
+
+
+def accuracy_fp32(dimhip, workloads, grid, steps):
+    r = accuracy_vs_fp64(dimhip, workloads.helium_shock_bubble_2d(*grid), steps,
+                         f"helium shock-bubble {grid[0]}x{grid[1]}")
+    r["note"] = ("open boundaries: mass changes by the boundary flux; the forms' difference is the FP32 form's "
+                 "extra drift (closed-box drift: tests/test_gpu_fp32_mixed.py::test_closed_box_mass_drift)")
+    return r
+
+
+def accuracy_batch(dimhip, workloads, count, steps, instances):
+    """The first batched instances, each run on its own in both forms (a
+    batched instance equals its single-grid FP32-mixed run bit for bit:
+    tests/test_gpu_batch.py::test_batch_equals_single_instances)."""
+    cases = workloads.batch_instances(count=count)[:instances]
+    per = []
+    for i, c in enumerate(cases):
+        r = accuracy_vs_fp64(dimhip, c, steps, f"instance {i} ({c.N[0]}x{c.N[1]})")
+        per.append({"instance": i, "grid": list(c.N), "time": r["time"], "l1_rho": r["l1_rho"], "l1_p": r["l1_p"],
+                    "l1_alpha0": r["l1_alpha0"], "fp64_noise_1e-7_l1": r["fp64_noise_1e-7_l1"],
+                    "mass_change_difference": r["mass_change_difference"]})
+    return {
+        "workload": f"instances 0..{instances - 1} of the batch, to the time of {steps} FP64 steps",
+        "instances": per,
     }
 
 
@@ -335,9 +389,19 @@ def main():
     if args.batch_steps > 0 and precision == "fp64":
         progress(f"batched {args.batch_count} instances: {args.batch_steps} steps")
         out["batched"] = measure_batch(dimhip, workloads, args.batch_count, args.batch_steps)
+        if args.accuracy_steps != 0:
+            progress(f"batched instances 0..3 fp32_mixed vs fp64 accuracy: {args.batch_steps} steps")
+            out["batched"]["accuracy_vs_fp64"] = accuracy_batch(dimhip, workloads, args.batch_count,
+                                                                args.batch_steps, 4)
     if args.c5_steps > 0 and precision == "fp64":
         progress(f"3D bubble collapse {args.c5_n}^3: {args.c5_steps} steps")
         out["bubble_collapse_3d"] = measure_3d(dimhip, workloads, args.c5_n, args.c5_steps, min(args.warmup, 5))
+        if args.accuracy_steps != 0 and args.c5_accuracy_n > 0:
+            n3 = args.c5_accuracy_n
+            progress(f"3D bubble collapse {n3}^3 fp32_mixed vs fp64 accuracy: {args.c5_accuracy_steps} steps")
+            out["bubble_collapse_3d"]["accuracy_vs_fp64"] = accuracy_vs_fp64(
+                dimhip, workloads.bubble_collapse_3d(n3, build_state=False), args.c5_accuracy_steps,
+                f"3D bubble collapse {n3}^3")
     if args.tube and precision == "fp64":
         progress("1D water/air tube full run")
         out["water_air_tube_1d"] = measure_tube(dimhip, workloads, not args.no_cpu_baseline)
