@@ -65,20 +65,21 @@ struct Flux {
 // ---- per-material EoS (src/euler/eos.hpp) --------------------------------
 // pressure<1> of a zero-momentum Conserved<1>: ideal and stiffened gas
 // (eos.hpp:451-466, 555-571) and noble-Abel (eos.hpp:665-677, whose
-// internal energy is internal_energy_from_conserved, eos.hpp:13-25)
+// internal energy is internal_energy_from_conserved, eos.hpp:13-25), for a
+// density r that is already clamp_min'ed (finite, >= 1e-10).  The
+// reference's kinetic energy 0.5 * safe_div(0, r) is then +0, and E - (+0)
+// is E bit for bit (signed zeros, infinities and NaN included), so the
+// division and the subtraction are not formed.
 template<typename EP>
-__device__ __forceinline__ double mat_pressure_c1(const EP& P, int k, double rho_in, double E)
+__device__ __forceinline__ double mat_pressure_c1_clamped(const EP& P, int k, double r, double E)
 {
-    const double rho = clamp_min(rho_in);
-    const double kinetic = 0.5 * safe_div(0.0, rho);
     if (EP::kNA && P.kind[k] == EOS_NOBLE_ABEL) {
-        const double e = smax(safe_div(E - kinetic, rho), 0.0);
-        const double denom = safe_denom(1.0 - P.covol[k] * rho, 1e-14);
-        return clamp_min(P.gm1[k] * rho * e / denom);
+        const double e = smax(safe_div(E, r), 0.0);
+        const double denom = safe_denom(1.0 - P.covol[k] * r, 1e-14);
+        return clamp_min(P.gm1[k] * r * e / denom);
     }
-    if (P.kind[k] == EOS_STIFFENED)
-        return clamp_min(P.gm1[k] * (E - kinetic) - P.gpinf[k]);
-    return clamp_min(P.gm1[k] * (E - kinetic));
+    if (P.kind[k] == EOS_STIFFENED) return clamp_min(P.gm1[k] * E - P.gpinf[k]);
+    return clamp_min(P.gm1[k] * E);
 }
 
 // internal_energy (eos.hpp:482-489, 587-595, 692-702); gm1 = gamma-1,
@@ -102,23 +103,24 @@ __device__ __forceinline__ double material_pressure_from_density_energy(const EP
                                                                          double rho, double e)
 {
     const double r = clamp_min(rho);
-    return mat_pressure_c1(P, k, r, r * smax(e, 0.0));
+    return mat_pressure_c1_clamped(P, k, r, r * smax(e, 0.0));
 }
 
 // eos.hpp:29-40 with sound_speed<1> (eos.hpp:469-479, 574-583, 680-689)
 template<typename EP>
 __device__ __forceinline__ double material_sound_speed(const EP& P, int k, double rho, double p)
 {
+    // clamp_min is idempotent: the reference's second clamp of r (in
+    // sound_speed<1> and pressure<1>) returns r
     const double r = clamp_min(rho);
     const double E = r * mat_internal_energy(P, k, r, p);
-    const double r2 = clamp_min(r);
-    const double pp = mat_pressure_c1(P, k, r, E);
+    const double pp = mat_pressure_c1_clamped(P, k, r, E);
     if (EP::kNA && P.kind[k] == EOS_NOBLE_ABEL) {
-        const double denom = safe_denom(r2 * (1.0 - P.covol[k] * r2), 1e-14);
+        const double denom = safe_denom(r * (1.0 - P.covol[k] * r), 1e-14);
         return __builtin_sqrt(clamp_min(P.gamma[k] * pp / denom));
     }
-    if (P.kind[k] == EOS_STIFFENED) return __builtin_sqrt(P.gamma[k] * safe_div(pp + P.pinf[k], r2));
-    return __builtin_sqrt(P.gamma[k] * safe_div(pp, r2));
+    if (P.kind[k] == EOS_STIFFENED) return __builtin_sqrt(P.gamma[k] * safe_div(pp + P.pinf[k], r));
+    return __builtin_sqrt(P.gamma[k] * safe_div(pp, r));
 }
 
 // eos.hpp:42-57

@@ -391,7 +391,8 @@ int dimhip_create(const dimhip_desc* desc, dimhip_ctx** out)
     if (!desc || !out) return fail(nullptr, "dimhip_create: null argument");
     std::string msg;
     if (validate_desc(desc, msg)) return fail(nullptr, msg);
-    auto c = std::make_unique<dimhip_ctx>();
+    // an early return releases whatever was allocated so far
+    std::unique_ptr<dimhip_ctx, int (*)(dimhip_ctx*)> c(new dimhip_ctx, dimhip_destroy);
     c->desc = *desc;
     for (int d = desc->dim; d < 3; ++d) {
         c->desc.N[d] = 1;
@@ -1189,7 +1190,8 @@ const char* dimhip_batch_last_error(const dimhip_batch* b) { return b ? b->err.c
 int dimhip_batch_create(const dimhip_desc* descs, int count, dimhip_batch** out)
 {
     if (!descs || !out || count <= 0) return bfail(nullptr, "dimhip_batch_create: bad argument");
-    auto b = std::make_unique<dimhip_batch>();
+    // an early return releases the instances and buffers created so far
+    std::unique_ptr<dimhip_batch, int (*)(dimhip_batch*)> b(new dimhip_batch, dimhip_batch_destroy);
     for (int i = 0; i < count; ++i) {
         const dimhip_desc& d = descs[i];
         if (d.dim != 2 || d.nmat != 2 || d.precision != DIMHIP_FP32_MIXED || d.time_update != DIMHIP_SPLIT)
@@ -1198,10 +1200,8 @@ int dimhip_batch_create(const dimhip_desc* descs, int count, dimhip_batch** out)
         for (int a = 0; a < 2; ++a)
             if (d.bc_lo[a] == DIMHIP_BC_NONREFLECTIVE || d.bc_hi[a] == DIMHIP_BC_NONREFLECTIVE) b->fusable = false;
         dimhip_ctx* c = nullptr;
-        if (dimhip_create(&d, &c)) {
-            for (auto* x : b->ctx) dimhip_destroy(x);
+        if (dimhip_create(&d, &c))
             return bfail(nullptr, std::string("instance ") + std::to_string(i) + ": " + g_global_err);
-        }
         b->ctx.push_back(c);
     }
     BHIP_OK(nullptr, hipSetDevice(descs[0].device));
