@@ -27,10 +27,9 @@ __device__ __forceinline__ unsigned long long double_to_bits(double x) { return 
 // EosParamsNA (also noble-Abel); Fp64 and Fp64NA below.
 template<bool NA>
 struct Fp64T {
-    // register cap of the sweep kernels: 3 waves per SIMD (168 VGPRs; the 2D
-    // sweep spills 28 B per lane, against 196 B at 4 waves, at the same speed)
+    // register cap of the sweep kernels: 4 waves per SIMD (128 VGPRs)
     template<int D, int M>
-    static constexpr int min_waves_per_eu() { return 3; }
+    static constexpr int min_waves_per_eu() { return 4; }
     // the sweep re-reads a cell's input state instead of keeping it in
     // registers across its phases (register pressure, not bandwidth, binds here)
     static constexpr bool kReloadInput = true;
