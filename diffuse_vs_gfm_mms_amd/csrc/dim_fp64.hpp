@@ -254,14 +254,21 @@ __device__ __forceinline__ double bisect_pressure(const EP& P, double target, do
         const double Cmt = C - target;
         const double B = Sabs * smax(hi, 1e-10) + Cabs + __builtin_fabs(target);  // pk <= max(hi, 1e-10) from here on
         const double thr = tol + 3.5527136788005009e-15 * B;                      // 32 u B
-        if (finite(S) && finite(Cmt) && B < 1e290) {
-            for (; n < 80; ++n) {
-                const double mid = 0.5 * (lo + hi);
-                const double e = __builtin_fma(S, smax(mid, 1e-10), Cmt);
-                if (!(__builtin_fabs(e) > thr)) break;
-                if (e > 0.0) hi = mid;
-                else lo = mid;
-            }
+        // The halvings run as a wave-uniform loop: each lane's updates are
+        // predicated on its own `go`, which falls permanently at its first
+        // uncertain halving, and n counts the halvings the lane took (its loop
+        // index on leaving the per-lane loop this replaces).  A lane takes at
+        // most one halving per pass, so the pass counter bounds n by 80.
+        bool go = finite(S) && finite(Cmt) && B < 1e290;
+        for (int pass = 0; pass < 80; ++pass) {
+            if (__builtin_amdgcn_ballot_w64(go) == 0) break;
+            const double mid = 0.5 * (lo + hi);
+            const double e = __builtin_fma(S, smax(mid, 1e-10), Cmt);
+            go = go && __builtin_fabs(e) > thr;
+            const bool up = e > 0.0;
+            hi = (go && up) ? mid : hi;
+            lo = (go && !up) ? mid : lo;
+            n += go ? 1 : 0;
         }
     }
     for (; n < 80; ++n) {
