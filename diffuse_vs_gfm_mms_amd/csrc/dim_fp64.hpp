@@ -259,16 +259,19 @@ __device__ __forceinline__ double bisect_pressure(const EP& P, double target, do
         // uncertain halving, and n counts the halvings the lane took (its loop
         // index on leaving the per-lane loop this replaces).  A lane takes at
         // most one halving per pass, so the pass counter bounds n by 80.
+        // lo and hi are finite here (hi passed the finite(fhi) test), so mid
+        // is finite and fmax(mid, 1e-10) is smax(mid, 1e-10); a lane still
+        // going at pass p has taken p + 1 halvings.
         bool go = finite(S) && finite(Cmt) && B < 1e290;
         for (int pass = 0; pass < 80; ++pass) {
             if (__builtin_amdgcn_ballot_w64(go) == 0) break;
             const double mid = 0.5 * (lo + hi);
-            const double e = __builtin_fma(S, smax(mid, 1e-10), Cmt);
+            const double e = __builtin_fma(S, __builtin_fmax(mid, 1e-10), Cmt);
             go = go && __builtin_fabs(e) > thr;
             const bool up = e > 0.0;
             hi = (go && up) ? mid : hi;
             lo = (go && !up) ? mid : lo;
-            n += go ? 1 : 0;
+            n = go ? pass + 1 : n;
         }
     }
     for (; n < 80; ++n) {
