@@ -53,8 +53,8 @@ batch_sweep_kernel(const BatchInst<Pol>* __restrict__ insts, const Work* __restr
     for (int w = beg; w < end; w += stride) {
         const Work it = work[w];
         const BatchInst<Pol>& I = insts[it.inst];
-        sweep_tile<Pol, 2, 2, DIR, L, W, 0, FUSE>(I.q[cur], I.q[cur ^ 1], I.g, I.P, I.ctl, I.cfl, nullptr, nullptr,
-                                                   it.a, it.b, 0);
+        sweep_tile<Pol, 2, 2, DIR, L, W, 0, FUSE, true>(I.q[cur], I.q[cur ^ 1], I.g, I.P, I.ctl, I.cfl, nullptr,
+                                                         nullptr, it.a, it.b, 0);
     }
 }
 

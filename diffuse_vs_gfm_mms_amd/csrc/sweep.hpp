@@ -117,19 +117,35 @@ __device__ __forceinline__ unsigned long long wave_reduce_u64(unsigned long long
     return ((unsigned long long)hi << 32) | lo;
 }
 
-// CFL reduction of one block into its dt slot (bit-ordered atomicMin of a
+// CFL reduction of one block into its dt slots (bit-ordered atomicMin of a
 // positive double).  A policy contributes either dt candidates combined by
 // min (the FP64 form: min_cell min_d cfl dx_d/(|u_d|+c), cfl.hpp:43-63) or
 // wave rates combined by max, inverted once per block (the FP32-mixed form).
 // Every thread of the block calls it.
-template<typename Pol, int NT>
+// PER_WAVE: each wave inverts its own result and issues its own atomic, with
+// no barrier or LDS; the min over waves of RN(1/r_w) is RN(1/max_w r_w), as
+// the reciprocal is monotone, so the slots end with the same minimum.  Used
+// by the persistent batched sweeps, whose blocks run many tiles: there the
+// two barriers of the block reduction cost more than the extra atomics.
+template<typename Pol, int NT, bool PER_WAVE = false>
 __device__ __forceinline__ void cfl_block_atomic(double v, StepCtl* ctl, int parity, int blk)
 {
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x + threadIdx.z * blockDim.x * blockDim.y;
+    if constexpr (PER_WAVE) {
+        constexpr int NW = NT / 64 > 0 ? NT / 64 : 1;
+        const unsigned long long w = wave_reduce_u64<Pol::kCflRate>(double_to_bits(v));
+        if ((tid & 63) == 0) {
+            unsigned long long* target = dt_slots(ctl, parity) + ((blk * NW + (tid >> 6)) % kDtSlots) * kDtStride;
+            const double r = bits_to_double(w);
+            const double dt = Pol::kCflRate ? (r > 0.0 ? 1.0 / r : bits_to_double(kInfBits)) : r;
+            if (dt < bits_to_double(kInfBits)) atomicMin(target, double_to_bits(dt));
+        }
+        return;
+    }
     unsigned long long* target = dt_slots(ctl, parity) + (blk % kDtSlots) * kDtStride;
     __shared__ unsigned long long red[NT / 64 > 0 ? NT / 64 : 1];
     __syncthreads();  // the block may reuse red[] for its next tile
     const unsigned long long w = wave_reduce_u64<Pol::kCflRate>(double_to_bits(v));
-    const int tid = threadIdx.x + threadIdx.y * blockDim.x + threadIdx.z * blockDim.x * blockDim.y;
     if ((tid & 63) == 0) red[tid >> 6] = w;
     __syncthreads();
     if (tid == 0) {
@@ -185,8 +201,9 @@ __device__ __forceinline__ void xcd_block(int& bx, int& by, int& bz)
 // stored by the CFL pass that read the same state; phase 1 takes it instead
 // of repeating the bisection.  pout (with FUSE_CFL): the recovered pressure
 // of every interior output cell, for the next sweep.
+// WAVE_CFL: the CFL epilogue reduces per wave (cfl_block_atomic's PER_WAVE).
 // ---------------------------------------------------------------------------
-template<typename Pol, int D, int M, int DIR, int L, int W, int MODE, bool FUSE_CFL>
+template<typename Pol, int D, int M, int DIR, int L, int W, int MODE, bool FUSE_CFL, bool WAVE_CFL = false>
 __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* __restrict__ qout, const Grid& g,
                                            const typename Pol::Eos& P, StepCtl* __restrict__ ctl,
                                            const CflArgs& cflargs, const double* __restrict__ pin,
@@ -433,7 +450,7 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
         }
     }
     if (FUSE_CFL)
-        cfl_block_atomic<Pol, S>(cand, ctl, ctl->slot ^ 1,
+        cfl_block_atomic<Pol, S, WAVE_CFL>(cand, ctl, ctl->slot ^ 1,
                                  (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)));
 }
 
