@@ -743,6 +743,8 @@ struct Fp32Mixed {
     template<int D, int M>
     static constexpr int min_waves_per_eu() { return (D <= 2 && M <= 2) ? 8 : 1; }
     static constexpr bool kReloadInput = false;
+    // the update's inputs from the first phase stay in registers
+    static constexpr bool kParkAlpha = false;
     // CFL reduction: each cell contributes its wave rate max_d (|u_d|+c)/(cfl dx_d),
     // combined by max in FP64; the block's dt is 1/rate (reciprocal is monotone,
     // so this equals the minimum of the per-cell 1/rate)

@@ -33,6 +33,10 @@ struct Fp64T {
     // the sweep re-reads a cell's input state instead of keeping it in
     // registers across its phases (register pressure, not bandwidth, binds here)
     static constexpr bool kReloadInput = true;
+    // the update's inputs from the first phase (P^n's volume fractions and
+    // the alpha source coefficients) wait in LDS, not registers, while the
+    // reconstruction and flux phases run (fewer spills at the 4-wave cap)
+    static constexpr bool kParkAlpha = true;
     // CFL reduction: each cell contributes its dt candidate, combined by min
     static constexpr bool kCflRate = false;
     // the CFL pass stores each cell's recovered pressure (cons_to_prim's p,

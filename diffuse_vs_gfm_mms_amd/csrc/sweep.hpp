@@ -218,7 +218,12 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     constexpr int NV = 2 * M + D;
     constexpr int NREC = 4 * M + 2 * D + 1;
     constexpr int T = L - 6;
-    __shared__ R sh[NREC * S];
+    // Pol::kParkAlpha: P^n's fractions and rhs_n wait in NPARK extra LDS rows
+    // of the thread's own slot from phase 1 to phase 5 (for nmat <= 2, whose
+    // tiles still fit the occupancy the register cap allows)
+    constexpr bool PARK = Pol::kParkAlpha && M <= 2;
+    constexpr int NPARK = PARK ? 2 * M : 0;
+    __shared__ R sh[(NREC + NPARK) * S];
 
     __syncthreads();  // a block may run several tiles (batched launches): LDS reuse
     const double dt = ctl->dt;
@@ -291,6 +296,13 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
         Pol::template enforce_c2p<D, M>(Pl);
         Pol::template alpha_rhs<D, M>(P, Pn, rhs_n);
         put_prim<R, D, M>(sh, S, slot, 0, Pl);
+        if constexpr (PARK) {
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                sh[(NREC + k) * S + slot] = Pn.alpha[k];
+                sh[(NREC + M + k) * S + slot] = rhs_n[k];
+            }
+        }
     }
     __syncthreads();
 
@@ -389,6 +401,13 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     // phase 5: update cell i
     double cand = cfl_identity<Pol>();
     if (own) {
+        if constexpr (PARK) {
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                Pn.alpha[k] = sh[(NREC + k) * S + slot];
+                rhs_n[k] = sh[(NREC + M + k) * S + slot];
+            }
+        }
         // with Pol::kReloadInput the cell's input state is read again (an L2
         // hit) rather than held in registers across the four phases
         Cell Uin;
