@@ -20,8 +20,8 @@
 
 namespace dimhip {
 
-constexpr int kSweepL0 = 128, kSweepW0 = 2;   // x sweep: 128 positions x 2 lines
-constexpr int kSweepL1 = 64, kSweepW1 = 8;    // y/z sweep: 8 lines (x) x 64 positions
+// sweep tiles come from the policy: Pol::kSweepL0 positions x Pol::kSweepW0
+// lines for x, Pol::kSweepW1 lines (along x) x Pol::kSweepL1 positions for y/z
 
 enum KernelClass { KC_SWEEP = 0, KC_BC = 1, KC_CFL = 2, KC_COUNT = 3 };
 

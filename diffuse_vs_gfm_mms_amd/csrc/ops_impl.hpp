@@ -14,17 +14,15 @@ void launch_sweep(dimhip_ctx* c, const void* in, void* out, const double* pin, d
 {
     const Grid& g = c->grid;
     if constexpr (DIR == 0) {
-        constexpr int T = kSweepL0 - 6;
-        dim3 blk(kSweepL0, kSweepW0, 1);
-        dim3 grd((g.N[0] + T - 1) / T, (g.N[1] + kSweepW0 - 1) / kSweepW0, g.N[2]);
-        sweep_kernel<Pol, D, M, 0, kSweepL0, kSweepW0, MODE, FUSE><<<grd, blk, 0, c->stream>>>(in, out, g, P, c->ctl, ca, pin,
-                                                                                                  pout);
+        constexpr int L = Pol::kSweepL0, W = Pol::kSweepW0, T = L - 6;
+        dim3 blk(L, W, 1);
+        dim3 grd((g.N[0] + T - 1) / T, (g.N[1] + W - 1) / W, g.N[2]);
+        sweep_kernel<Pol, D, M, 0, L, W, MODE, FUSE><<<grd, blk, 0, c->stream>>>(in, out, g, P, c->ctl, ca, pin, pout);
     } else {
-        constexpr int T = kSweepL1 - 6;
-        dim3 blk(kSweepW1, kSweepL1, 1);
-        dim3 grd((g.N[0] + kSweepW1 - 1) / kSweepW1, (g.N[DIR] + T - 1) / T, DIR == 1 ? g.N[2] : g.N[1]);
-        sweep_kernel<Pol, D, M, DIR, kSweepL1, kSweepW1, MODE, FUSE><<<grd, blk, 0, c->stream>>>(in, out, g, P, c->ctl, ca, pin,
-                                                                                                    pout);
+        constexpr int L = Pol::kSweepL1, W = Pol::kSweepW1, T = L - 6;
+        dim3 blk(W, L, 1);
+        dim3 grd((g.N[0] + W - 1) / W, (g.N[DIR] + T - 1) / T, DIR == 1 ? g.N[2] : g.N[1]);
+        sweep_kernel<Pol, D, M, DIR, L, W, MODE, FUSE><<<grd, blk, 0, c->stream>>>(in, out, g, P, c->ctl, ca, pin, pout);
     }
 }
 

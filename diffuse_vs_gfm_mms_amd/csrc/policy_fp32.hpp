@@ -745,6 +745,8 @@ struct Fp32Mixed {
     static constexpr bool kReloadInput = false;
     // the update's inputs from the first phase stay in registers
     static constexpr bool kParkAlpha = false;
+    // sweep tiles: x 128 positions x 2 lines; y/z 8 lines x 64 positions
+    static constexpr int kSweepL0 = 128, kSweepW0 = 2, kSweepL1 = 64, kSweepW1 = 8;
     // CFL reduction: each cell contributes its wave rate max_d (|u_d|+c)/(cfl dx_d),
     // combined by max in FP64; the block's dt is 1/rate (reciprocal is monotone,
     // so this equals the minimum of the per-cell 1/rate)

@@ -37,6 +37,9 @@ struct Fp64T {
     // the alpha source coefficients) wait in LDS, not registers, while the
     // reconstruction and flux phases run (fewer spills at the 4-wave cap)
     static constexpr bool kParkAlpha = true;
+    // sweep tiles: x 128 positions x 2 lines; y/z 4 lines x 128 positions
+    // (halo 6 of 128; 8 lines x 64 positions was 1.6 % slower)
+    static constexpr int kSweepL0 = 128, kSweepW0 = 2, kSweepL1 = 128, kSweepW1 = 4;
     // CFL reduction: each cell contributes its dt candidate, combined by min
     static constexpr bool kCflRate = false;
     // the CFL pass stores each cell's recovered pressure (cons_to_prim's p,
