@@ -201,9 +201,12 @@ __device__ __forceinline__ void xcd_block(int& bx, int& by, int& bz)
 // stored by the CFL pass that read the same state; phase 1 takes it instead
 // of repeating the bisection.  pout (with FUSE_CFL): the recovered pressure
 // of every interior output cell, for the next sweep.
-// WAVE_CFL: the CFL epilogue reduces per wave (cfl_block_atomic's PER_WAVE).
+// PERSISTENT: the block runs many tiles (the batched launches).  Its CFL
+// epilogue then reduces per wave (cfl_block_atomic's PER_WAVE), and no LDS
+// rows are added for Pol::kParkAlpha, since the batched autotuner packs
+// several blocks per CU.
 // ---------------------------------------------------------------------------
-template<typename Pol, int D, int M, int DIR, int L, int W, int MODE, bool FUSE_CFL, bool WAVE_CFL = false>
+template<typename Pol, int D, int M, int DIR, int L, int W, int MODE, bool FUSE_CFL, bool PERSISTENT = false>
 __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* __restrict__ qout, const Grid& g,
                                            const typename Pol::Eos& P, StepCtl* __restrict__ ctl,
                                            const CflArgs& cflargs, const double* __restrict__ pin,
@@ -221,7 +224,7 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     // Pol::kParkAlpha: P^n's fractions and rhs_n wait in NPARK extra LDS rows
     // of the thread's own slot from phase 1 to phase 5 (for nmat <= 2, whose
     // tiles still fit the occupancy the register cap allows)
-    constexpr bool PARK = Pol::kParkAlpha && M <= 2;
+    constexpr bool PARK = Pol::kParkAlpha && M <= 2 && !PERSISTENT;
     constexpr int NPARK = PARK ? 2 * M : 0;
     __shared__ R sh[(NREC + NPARK) * S];
 
@@ -469,7 +472,7 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
         }
     }
     if (FUSE_CFL)
-        cfl_block_atomic<Pol, S, WAVE_CFL>(cand, ctl, ctl->slot ^ 1,
+        cfl_block_atomic<Pol, S, PERSISTENT>(cand, ctl, ctl->slot ^ 1,
                                  (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)));
 }
 
