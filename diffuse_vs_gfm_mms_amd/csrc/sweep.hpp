@@ -314,6 +314,14 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     Pr Ph;
     const bool ph2 = in && pos >= 1 && pos < L - 1;
     if (ph2) {
+        if constexpr (Pol::kReloadInput) {
+            // the cell's state and P_line are read back (global, an L2 hit;
+            // the thread's own LDS slot) instead of held across the barrier
+            Cell U2;
+            Pol::template load<D, M>(qin, g.pitch, cell, U2);
+            Pol::template to_state<D, M>(U2, U);
+            get_prim<R, D, M>(sh, S, slot, 0, Pl);
+        }
         if (end_cell) {
             Uh = U;
             Ph = Pl;
