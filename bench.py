@@ -9,7 +9,9 @@ value = cells x steps / wall seconds of the timed region (the reference's
 metric, tools/quant/report_builders.hpp:298-300).  roofline: the sweep
 kernel's algorithmic bytes per launch (each directional sweep reads and
 writes the nv conserved fields once: 2 * nv * bytes/value per cell) over its
-mean device duration from HIP events on the library's stream.  cpu_baseline:
+mean device duration from HIP events on the library's stream, recorded
+around the kernels of one step in TIMING_EVERY of the timed region (events on
+every step cost 4-8 % of the throughput they would measure).  cpu_baseline:
 the reference's own OpenMP solver (oracle/_ref/libdimref.so, built from the
 reference's sources) timed on this host on a bounded sample.
 """
@@ -25,6 +27,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# event timing samples one step in this many inside the timed region
+TIMING_EVERY = 10
 
 WORKLOADS = {
     # name: (grid, precision, value bytes)
@@ -112,7 +116,7 @@ def measure(dimhip, workloads, grid, precision, steps, warmup):
     solver = dimhip.Solver.from_case(case, precision=precision)
     t, n = solver.advance(warmup, case.cfl, t_end, 0.0, use_graph=False)
     solver.synchronize()
-    solver.set_timing(True)
+    solver.set_timing(True, every=TIMING_EVERY)
     t0 = time.perf_counter()
     t, n = solver.advance(steps, case.cfl, t_end, t, use_graph=False)
     solver.synchronize()
@@ -220,7 +224,7 @@ def measure_3d(dimhip, workloads, n, steps, warmup):
     with dimhip.Solver.from_case(case, precision="fp32_mixed") as s:
         t, _ = s.advance(warmup, case.cfl, 1.0, 0.0, use_graph=False)
         s.synchronize()
-        s.set_timing(True)
+        s.set_timing(True, every=TIMING_EVERY)
         t0 = time.perf_counter()
         t, nt = s.advance(steps, case.cfl, 1.0, t, use_graph=False)
         s.synchronize()
@@ -360,6 +364,7 @@ def main():
             "kernel": "sweep (x and y launches averaged)",
             "mean_ms": {"sweep": ms[0], "boundary": ms[1], "cfl": ms[2]},
             "launches": {"sweep": counts[0], "boundary": counts[1], "cfl": counts[2]},
+            "timed_steps": f"1 in {TIMING_EVERY}",
             "bytes_per_launch": bytes_per_launch,
         },
         "cpu_baseline": None,

@@ -74,8 +74,11 @@ struct dimhip_ctx {
     hipGraph_t graph = nullptr;
     double graph_cfl = -1.0;
     int graph_cur = -1;
-    // event timing
-    bool timing = false;
+    // event timing: 0 off; k > 0 records events around the kernels of one
+    // step in k (timing_step counts the steps enqueued since it was set)
+    int timing = 0;
+    long long timing_step = 0;
+    bool timing_now = false;
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, int>> ev_pending;  // (class, index of start event)
     size_t ev_next = 0;

@@ -47,7 +47,7 @@ int fail(dimhip_ctx* ctx, const std::string& msg)
 // ---- event timing ---------------------------------------------------------
 int ev_begin(dimhip_ctx* c)
 {
-    if (!c->timing) return -1;
+    if (!c->timing_now) return -1;
     if (c->ev_next + 2 > c->ev_pool.size()) {
         for (int k = 0; k < 64; ++k) {
             hipEvent_t e;
@@ -286,6 +286,7 @@ void enqueue_step(dimhip_ctx* c, double cfl, double dt_max_host)
 {
     const bool unsplit = c->desc.time_update == DIMHIP_UNSPLIT;
     const int D = c->desc.dim;
+    c->timing_now = c->timing > 0 && c->timing_step++ % c->timing == 0;
     // The CFL pass of the step's input state (the previous step's fused
     // epilogue, or the stand-alone kernel below) leaves its recovered
     // pressures in pstore (FP64, dim >= 2): the sweeps that read that state
@@ -873,12 +874,14 @@ int dimhip_synchronize(dimhip_ctx* c)
     return 0;
 }
 
-int dimhip_set_timing(dimhip_ctx* c, int enable)
+int dimhip_set_timing(dimhip_ctx* c, int every)
 {
     if (!c) return fail(c, "dimhip_set_timing: null context");
     HIP_OK(c, hipStreamSynchronize(c->stream));
     ev_collect(c);
-    c->timing = enable != 0;
+    c->timing = every > 0 ? every : 0;
+    c->timing_step = 0;
+    c->timing_now = false;
     for (int k = 0; k < KC_COUNT; ++k) {
         c->kt_sum[k] = 0.0;
         c->kt_cnt[k] = 0;

@@ -352,8 +352,9 @@ class Solver:
     def synchronize(self):
         self._check(self.lib.dimhip_synchronize(self.h))
 
-    def set_timing(self, on):
-        self._check(self.lib.dimhip_set_timing(self.h, int(on)))
+    def set_timing(self, on, every=1):
+        """Event timing of the kernels of one step in `every` (off when `on` is false)."""
+        self._check(self.lib.dimhip_set_timing(self.h, int(every) if on else 0))
 
     def kernel_times(self):
         """Mean device ms per launch of (sweep, boundary, cfl) kernels and launch counts."""

@@ -219,8 +219,12 @@ int dimhip_synchronize(dimhip_ctx* ctx);
 /* Mean device duration (ms) of each kernel class over the calls since the
  * last reset, measured with HIP events on the context's stream:
  * out[0] sweep kernels, out[1] boundary kernels, out[2] CFL kernels;
- * counts[i] = launches measured.  Timing is enabled by dimhip_set_timing. */
-int dimhip_set_timing(dimhip_ctx* ctx, int enable);
+ * counts[i] = launches measured.  dimhip_set_timing(ctx, every) resets the
+ * sums and, for every > 0, records events around the kernels of one step in
+ * `every` (1: every step; sampling keeps the events' own cost out of the
+ * other steps); every <= 0 turns timing off.  Graph replay is off while
+ * timing is on. */
+int dimhip_set_timing(dimhip_ctx* ctx, int every);
 int dimhip_kernel_times(dimhip_ctx* ctx, double* mean_ms, long long* counts);
 
 /* Known-answer entry point for the device functions on the path.  `which`:
