@@ -184,6 +184,29 @@ def test_device_loop_and_graph_match_host_steps():
         assert bits_equal(dev, host), (use_graph, mismatch_report(dev, host))
 
 
+@pytest.mark.parametrize("every", [1, 3, 4])
+def test_sampled_kernel_timing(every):
+    """dimhip_set_timing(ctx, k) times the kernels of one step in k and leaves
+    the state unchanged: 9 steps (after the first, whose CFL pass runs on its
+    own) give ceil(8 / k) timed steps of 2 sweeps and 2 boundary passes."""
+    c = W.helium_shock_bubble_2d(128, 64)
+    with dimhip.Solver.from_case(c) as s:
+        t, _ = s.advance(1, c.cfl, 1.0, 0.0, use_graph=False)
+        s.set_timing(True, every=every)
+        t, n = s.advance(8, c.cfl, 1.0, t, use_graph=False)
+        ms, counts = s.kernel_times()
+        timed = s.download()
+        s.set_timing(False)
+    with dimhip.Solver.from_case(c) as s:
+        s.advance(9, c.cfl, 1.0, 0.0, use_graph=False)
+        plain = s.download()
+    steps = -(-8 // every)
+    assert n == 8
+    assert counts == [2 * steps, 2 * steps, 0]
+    assert ms[0] > 0.0 and ms[1] > 0.0
+    assert bits_equal(timed, plain), mismatch_report(timed, plain)
+
+
 def test_device_loop_stops_at_t_end(oracle):
     """The loop ends once time >= t_end - 1e-14 and the last dt lands on t_end."""
     c = W.water_air_tube_1d(120)
