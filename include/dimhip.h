@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define DIMHIP_VERSION 2
+#define DIMHIP_VERSION 3
 #define DIMHIP_MAX_MAT 4
 
 /* EoS kinds on the path (src/euler/eos_params.hpp:6-13; the config names
