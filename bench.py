@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--c5-accuracy-n", type=int, default=256,
                     help="grid of the 3D FP32-mixed vs FP64 comparison (0: skip)")
     ap.add_argument("--c5-accuracy-steps", type=int, default=100)
+    ap.add_argument("--next-steps", type=int, default=100,
+                    help="steps of each widened-row measurement (noble-Abel, unsplit, lambda model, "
+                         "nmat 3; 0: skip)")
     ap.add_argument("--tube", action=argparse.BooleanOptionalAction, default=True,
                     help="the 1D water/air tube full run on the GPU and in the reference's CPU solver")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -246,6 +249,59 @@ def measure_3d(dimhip, workloads, n, steps, warmup):
     }
 
 
+def measure_next_rows(dimhip, workloads, steps):
+    """The path's widened rows, FP64, each timed over `steps` steps after 5
+    warm-up steps (no events): the noble-Abel water/air bubble (Gorsse TC9),
+    the unsplit update, the equal-velocity lambda model, a third material
+    slot, and the device-side output rows and initial state."""
+    def run(case, label, **kw):
+        with dimhip.Solver.from_case(case, precision="fp64", **kw) as s:
+            t, _ = s.advance(5, case.cfl, 1.0, 0.0, use_graph=False)
+            s.synchronize()
+            t0 = time.perf_counter()
+            t, n = s.advance(steps, case.cfl, 1.0, t, use_graph=False)
+            s.synchronize()
+            wall = time.perf_counter() - t0
+        if n != steps:
+            raise RuntimeError(f"{label}: took {n} steps, expected {steps}")
+        return {"workload": label, "steps": n, "ms_per_step": wall / n * 1e3,
+                "value": case.ncells * n / wall, "unit": "cell-updates/s"}
+
+    out = {}
+    c = workloads.gorsse_tc9_2d(960, 800, build_state=False)
+    out["noble_abel_tc9_2d"] = run(c, "Gorsse TC9 water (stiffened) / air (noble-Abel) bubble 960x800, "
+                                      "device initial state, CFL 0.45")
+    c = workloads.helium_shock_bubble_2d(2048, 1024)
+    out["unsplit_2d"] = run(c, "helium shock-bubble 2048x1024, unsplit update", time_update="unsplit")
+    c.lambda_model = "equal_velocity"
+    out["equal_velocity_2d"] = run(c, "helium shock-bubble 2048x1024, equal-velocity lambda model")
+    c3 = workloads.helium_shock_bubble_2d(2048, 1024)
+    c3.materials = c3.materials + [workloads.WATER]
+    c3.U = None  # three-material state from the device initialiser (water slot empty)
+    out["three_material_2d"] = run(c3, "helium shock-bubble 2048x1024 with a third (water) material "
+                                       "slot, empty, nmat = 3 kernels")
+    with dimhip.Solver.from_case(c, precision="fp64") as s:
+        s.advance(5, c.cfl, 1.0, 0.0, use_graph=False)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.output_rows()
+        wall = time.perf_counter() - t0
+    out["output_rows_2d"] = {"workload": "CSV row values of helium 2048x1024 computed on the device "
+                                         "(FP64 primitives, energies, per-material fields) and copied "
+                                         "to the host", "seconds": wall, "rows_per_s": c.ncells / wall}
+    c = workloads.bubble_collapse_3d(512, build_state=False)
+    with dimhip.Solver(c.dim, c.N, c.dx, c.materials, c.bc_lo, c.bc_hi, c.lambda_model,
+                       c.alpha_source_floor, "split", "fp32_mixed") as s:
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.initialise(c.ic)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+    out["device_ic_3d"] = {"workload": "3D bubble collapse 512^3 initial state built on the device",
+                           "seconds": wall, "cells_per_s": c.ncells / wall}
+    return out
+
+
 def measure_tube(dimhip, workloads, cpu):
     """The 1D water/air tube (1000 cells) run to t = 2.4e-4 s in the FP64 form
     with graph-captured steps; the reference's solver takes the same number of
@@ -407,6 +463,9 @@ def main():
             out["bubble_collapse_3d"]["accuracy_vs_fp64"] = accuracy_vs_fp64(
                 dimhip, workloads.bubble_collapse_3d(n3, build_state=False), args.c5_accuracy_steps,
                 f"3D bubble collapse {n3}^3")
+    if args.next_steps > 0 and precision == "fp64":
+        progress(f"widened rows: {args.next_steps} FP64 steps each")
+        out["next_rows"] = measure_next_rows(dimhip, workloads, args.next_steps)
     if args.tube and precision == "fp64":
         progress("1D water/air tube full run")
         out["water_air_tube_1d"] = measure_tube(dimhip, workloads, not args.no_cpu_baseline)
