@@ -27,9 +27,11 @@ __device__ __forceinline__ unsigned long long double_to_bits(double x) { return 
 // EosParamsNA (also noble-Abel); Fp64 and Fp64NA below.
 template<bool NA>
 struct Fp64T {
-    // register cap of the sweep kernels: 4 waves per SIMD (128 VGPRs)
+    // register cap of the sweep kernels: 4 waves per SIMD (128 VGPRs); the
+    // three-material kernels (without noble-Abel) run 6 % faster at 2 waves
+    // (256 VGPRs) than with the 700 B per lane of spills the 4-wave cap costs
     template<int D, int M>
-    static constexpr int min_waves_per_eu() { return 4; }
+    static constexpr int min_waves_per_eu() { return (M >= 3 && !NA) ? 2 : 4; }
     // the sweep re-reads a cell's input state instead of keeping it in
     // registers across its phases (register pressure, not bandwidth, binds here)
     static constexpr bool kReloadInput = true;
