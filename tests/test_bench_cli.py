@@ -1,0 +1,56 @@
+"""bench.py host logic (no GPU): the one-GPU contract, the algorithmic bytes
+behind roofline.achieved, and the committed traffic summaries behind
+roofline.traffic."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+from diffuse_vs_gfm_mms_amd import workloads as W  # noqa: E402
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    assert line["n_gpus"] == 2 and "one GPU" in line["error"]
+
+
+@pytest.mark.parametrize("precision, grid, expected", [
+    # FP64: 6 fields x 8 B, read and written: 96 B per cell
+    ("fp64", (2048, 1024), 96 * 2048 * 1024),
+    # FP32-mixed: 2 FP64 partial masses + 4 FP32 fields, read and written: 64 B per cell
+    ("fp32_mixed", (4096, 2048), 64 * 4096 * 2048),
+])
+def test_sweep_bytes(precision, grid, expected):
+    case = W.helium_shock_bubble_2d(16, 8)
+    case.N = grid  # only the cell count and the field counts enter
+    assert bench.sweep_bytes(case, precision) == expected
+
+
+def test_3d_sweep_bytes():
+    case = W.bubble_collapse_3d(8, build_state=False)
+    case.N = (512, 512, 512)
+    # FP32-mixed 3D: 2 x (2 x 8 B + 5 x 4 B) per cell
+    assert bench.sweep_bytes(case, "fp32_mixed") == 2 * (2 * 8 + 5 * 4) * 512 ** 3
+
+
+@pytest.mark.parametrize("workload, algorithmic", [("helium_fp64", 201326592), ("helium_fp32", 536870912)])
+def test_traffic_summaries(workload, algorithmic):
+    traffic = bench.traffic_for(workload)
+    assert isinstance(traffic, int)
+    with open(os.path.join(ROOT, "profiles", f"traffic_{workload}.json")) as f:
+        t = json.load(f)
+    assert t["algorithmic_bytes_per_sweep_launch"] == algorithmic
+    assert t["bytes_per_sweep_launch"] == traffic
+    # the mean of the x and y launches, and at least the algorithmic bytes
+    assert abs(traffic - (t["x_sweep_bytes"] + t["y_sweep_bytes"]) / 2) <= 1
+    assert traffic >= algorithmic
+    assert abs(t["ratio_to_algorithmic"] - traffic / algorithmic) < 1e-3
