@@ -69,6 +69,9 @@ def parse():
     ap.add_argument("--tube", action=argparse.BooleanOptionalAction, default=True,
                     help="the 1D water/air tube full run on the GPU and in the reference's CPU solver")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sweep-probe", choices=["fp64", "fp32_mixed"], default=None,
+                    help="profiling aid: 5 steps of the helium workload, then 30 single unfused x sweeps and "
+                         "30 single unfused y sweeps (the fused y sweep runs inside the steps)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="N",
                     help="only run a device copy of N doubles (calibrates HBM byte counters) and exit")
     return ap.parse_args()
@@ -370,6 +373,19 @@ def main():
         sys.exit(2)
     from diffuse_vs_gfm_mms_amd import dimhip, workloads
 
+    if args.sweep_probe:
+        grid = WORKLOADS["helium_fp64" if args.sweep_probe == "fp64" else "helium_fp32"][0]
+        case = workloads.helium_shock_bubble_2d(*grid)
+        with dimhip.Solver.from_case(case, precision=args.sweep_probe) as s:
+            t, _ = s.advance(5, case.cfl, 1.0, 0.0, use_graph=False)
+            dt = s.compute_dt(case.cfl, 1.0) * 1e-3  # a small step: the state barely moves
+            for d in (0, 1):
+                for _ in range(30):
+                    s.sweep(d, dt)
+            s.synchronize()
+        print(json.dumps({"sweep_probe": args.sweep_probe, "grid": list(grid), "steps": 5,
+                          "unfused_sweeps_per_axis": 30}))
+        return
     if args.copy_probe:
         import numpy as np
         src = np.arange(args.copy_probe, dtype=np.float64)
