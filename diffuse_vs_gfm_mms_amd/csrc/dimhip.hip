@@ -425,11 +425,17 @@ int dimhip_create(const dimhip_desc* desc, dimhip_ctx** out)
         c->eosm.inv_gm1[k] = (float)c->eosm.inv_gm1_d[k];
         c->eosm.b[k] = (float)c->eosm.b_d[k];
         c->eosm.bmin[k] = (float)(1e-10 / c->eos.gm1[k]);
+        c->eosm.gm1[k] = (float)c->eos.gm1[k];
+        c->eosm.coef_lo[k] = std::min(1.0f, c->eosm.inv_gm1[k]);
+        c->eosm.b_ghost[k] = c->eos.gamma[k] < 2.0 ? c->eosm.b[k] : 0.0f;
     }
     c->eosm.lambda_model = desc->lambda_model;
     c->eosm.stiff = 0;
-    for (int k = 0; k < desc->nmat; ++k)
+    c->eosm.stiff_ghost = 0;
+    for (int k = 0; k < desc->nmat; ++k) {
         if (desc->eos_kind[k] == DIMHIP_EOS_STIFFENED) c->eosm.stiff = 1;
+        if (c->eosm.b_ghost[k] != 0.0f) c->eosm.stiff_ghost = 1;
+    }
     c->eosm.alpha_source_floor = (float)desc->alpha_source_floor;
     // ring cells per face (d, side), owned by the face of their lowest boundary axis
     long long acc = 0;

@@ -34,9 +34,13 @@ struct EosMixed {
     float bmin[kMaxMat];      // 1e-10/(gamma-1): shifted energy floor per unit volume fraction
     double inv_gm1_d[kMaxMat];
     double b_d[kMaxMat];
+    float gm1[kMaxMat];       // gamma - 1
+    float coef_lo[kMaxMat];   // min(1, 1/(gamma-1)): lower bound of ecoef
+    float b_ghost[kMaxMat];   // b for gamma < 2 (where a near-massless fraction is down-weighted), else 0
     int lambda_model;
     float alpha_source_floor;
     int stiff;                // any stiffened-gas material: B(alpha) is not identically zero
+    int stiff_ghost;          // any b_ghost != 0
 };
 
 namespace f32 {
@@ -57,6 +61,44 @@ __device__ __forceinline__ float safe_denom_f(float d, float tol = 1e-10f)
 __device__ __forceinline__ float safe_div_f(float n, float d, float tol = 1e-10f)
 {
     return n * rcp(safe_denom_f(d, tol));
+}
+
+// A material's weight in the mixture energy per unit volume fraction and
+// pressure, w_k/(gamma_k - 1).  The reference evaluates each material law at
+// clamp_min(rho_k) (>= 1e-10) and divides by safe_denom((gamma_k - 1) rho_k)
+// (>= 1e-10) (src/dim/eos.hpp:59-75, src/euler/eos.hpp:607-614), so a
+// fraction whose partial density is below 1e-10/(gamma_k - 1) -- one that
+// carries (almost) no mass, as inactive-mass removal leaves behind
+// (primitives.hpp:392-399) -- enters with w_k = (gamma_k - 1)
+// max(rho_k, 1e-10)/1e-10 < 1 instead of 1:
+//   w_k/(gamma_k - 1) = min(1/(gamma_k - 1), max(1e10 rho_k, 1))
+//                     = med3(1e10 rho_k, min(1, 1/(gamma_k - 1)), 1/(gamma_k - 1)).
+// Materials with gamma_k >= 2 always have w_k = 1.
+__device__ __forceinline__ float ecoef(const EosMixed& P, int k, float rho_k)
+{
+    return __builtin_amdgcn_fmed3f(rho_k * 1e10f, P.coef_lo[k], P.inv_gm1[k]);
+}
+
+// rho_k c_k^2 of the reference's material_sound_speed (src/dim/eos.hpp:29-40):
+// the pressure recomputed from the energy built at the clamped density,
+// p' = clamp_min(w_k (p + gamma p_inf) - gamma p_inf), and
+// rho' c^2 = gamma (p' + p_inf); coef = ecoef(rho_k)
+__device__ __forceinline__ float impedance(const EosMixed& P, int k, float p, float coef)
+{
+    const float w = coef * P.gm1[k];
+    if (P.stiff) return P.gamma[k] * (fmaxs(w * (p + P.gpinf[k]) - P.gpinf[k], 1e-10f) + P.pinf[k]);
+    return P.gamma[k] * fmaxs(w * p, 1e-10f);
+}
+
+// sum_k alpha_k (1 - w_k) b_k: the stiffening energy a down-weighted fraction
+// does not carry (only for stiffened materials with gamma < 2)
+template<int M>
+__device__ __forceinline__ float ghost_shift(const EosMixed& P, const float* alpha, const float* coef)
+{
+    float s = -0.f;
+#pragma unroll
+    for (int k = 0; k < M; ++k) s += fmaxs(alpha[k], 0.f) * P.b_ghost[k] * (1.f - coef[k] * P.gm1[k]);
+    return s;
 }
 
 template<int D, int M>
@@ -215,33 +257,29 @@ __device__ __forceinline__ bool interface_cell(const float* alpha)
     return itf;
 }
 
-// p from the shifted internal energy density (rho e - B(alpha)) = sum_k alpha_k p/(gamma_k-1)
+// p from the shifted internal energy density
+// (rho e - B(alpha)) = sum_k alpha_k w_k p/(gamma_k-1) - sum_k alpha_k (1 - w_k) b_k,
+// coef[k] = ecoef of the material's partial density
 template<int M>
-__device__ __forceinline__ float pressure_shifted(const EosMixed& P, float rho_e_hat, const float* alpha)
+__device__ __forceinline__ float pressure_shifted(const EosMixed& P, float rho_e_hat, const float* alpha,
+                                                  const float* coef)
 {
     float S = -0.f;
 #pragma unroll
-    for (int k = 0; k < M; ++k) S += fmaxs(alpha[k], 0.f) * P.inv_gm1[k];
+    for (int k = 0; k < M; ++k) S += fmaxs(alpha[k], 0.f) * coef[k];
+    if (P.stiff_ghost) rho_e_hat += ghost_shift<M>(P, alpha, coef);
     return rho_e_hat * rcp(S);
 }
 
+// rho e - B at pressure p: sum_k alpha_k w_k max(p,1e-10)/(gamma_k-1) - sum_k alpha_k (1 - w_k) b_k
 template<int M>
-__device__ __forceinline__ float pressure_shifted_d(const EosMixed& P, double rho_e_hat, const float* alpha)
-{
-    double S = -0.0;
-#pragma unroll
-    for (int k = 0; k < M; ++k) S += (double)fmaxs(alpha[k], 0.f) * P.inv_gm1_d[k];
-    return (float)(rho_e_hat / S);
-}
-
-// sum_k alpha_k max(p,1e-10)/(gamma_k-1): rho e - B at pressure p
-template<int M>
-__device__ __forceinline__ float energy_shifted(const EosMixed& P, float p, const float* alpha)
+__device__ __forceinline__ float energy_shifted(const EosMixed& P, float p, const float* alpha, const float* coef)
 {
     const float pk = fmaxs(p, 1e-10f);
     float s = -0.f;
 #pragma unroll
-    for (int k = 0; k < M; ++k) s += fmaxs(alpha[k], 0.f) * (pk * P.inv_gm1[k]);
+    for (int k = 0; k < M; ++k) s += fmaxs(alpha[k], 0.f) * (pk * coef[k]);
+    if (P.stiff_ghost) s -= ghost_shift<M>(P, alpha, coef);
     return s;
 }
 
@@ -266,8 +304,12 @@ __device__ __forceinline__ void cons_to_prim(const EosMixed& P, const St<D, M>& 
         R.vel[d] = U.mom[d] * inv_rho;
         v2 += R.vel[d] * R.vel[d];
     }
+    float coef[M];
 #pragma unroll
-    for (int k = 0; k < M; ++k) R.rho[k] = fmaxs(U.m[k] * rcp(fmaxs(R.alpha[k], 1e-10f)), 1e-12f);
+    for (int k = 0; k < M; ++k) {
+        R.rho[k] = fmaxs(U.m[k] * rcp(fmaxs(R.alpha[k], 1e-10f)), 1e-12f);
+        coef[k] = ecoef(P, k, R.rho[k]);
+    }
     const float kinetic = 0.5f * rho * v2;
     const float shift = P.stiff ? bshift<M>(P, ad, R.alpha) : 0.f;
     float p;
@@ -282,9 +324,9 @@ __device__ __forceinline__ void cons_to_prim(const EosMixed& P, const St<D, M>& 
         double inv = (double)inv_rho;
         inv = inv * (2.0 - rd * inv);
         const double rho_e_hat = (double)U.E + (double)shift - 0.5 * m2 * inv;
-        p = pressure_shifted<M>(P, (float)rho_e_hat, R.alpha);
+        p = pressure_shifted<M>(P, (float)rho_e_hat, R.alpha, coef);
     } else {
-        p = pressure_shifted<M>(P, U.E + shift - kinetic, R.alpha);
+        p = pressure_shifted<M>(P, U.E + shift - kinetic, R.alpha, coef);
     }
     R.p = fmaxs(p, 1e-10f);
 }
@@ -298,10 +340,12 @@ __device__ __forceinline__ void prim_to_cons(const EosMixed& P, const Pr<D, M>& 
 #pragma unroll
     for (int k = 0; k < M; ++k) alpha[k] = Q.alpha[k];
     floor_fractions<M>(alpha);
-    float rho = -0.f;
+    float rho = -0.f, coef[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-        U.m[k] = alpha[k] * fmaxs(Q.rho[k], 1e-12f);
+        const float rk = fmaxs(Q.rho[k], 1e-12f);
+        U.m[k] = alpha[k] * rk;
+        coef[k] = ecoef(P, k, rk);
         rho += U.m[k];
     }
     rho = fmaxs(rho, 1e-12f);
@@ -318,7 +362,7 @@ __device__ __forceinline__ void prim_to_cons(const EosMixed& P, const Pr<D, M>& 
         decode_alpha<M>(U.a, ad);
         shift = bshift<M>(P, alpha, ad);
     }
-    U.E = energy_shifted<M>(P, fmaxs(Q.p, 1e-12f), alpha) + shift + 0.5f * rho * v2;
+    U.E = energy_shifted<M>(P, fmaxs(Q.p, 1e-12f), alpha, coef) + shift + 0.5f * rho * v2;
 }
 
 // repair_state (primitives.hpp:326-457) on a state whose partial masses are
@@ -394,8 +438,18 @@ __device__ __forceinline__ void repair(const EosMixed& P, S& U)
             U.mom[d] = rho * v;
             v2b += v * v;
         }
-        const float p_before = fmaxs(pressure_shifted<M>(P, e_hat_before - 0.5f * rho_before * v2b, alpha), 1e-10f);
-        U.E = energy_shifted<M>(P, p_before, alpha) + fix + 0.5f * rho * v2b;
+        // material densities before and after the removal, as the reference's
+        // rho_material_before / rho_material (primitives.hpp:352-360, 404-410)
+        float cb[M], ca[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const float ia = rcp(fmaxs(alpha[k], 1e-10f));
+            cb[k] = ecoef(P, k, fmaxs((float)m_in[k] * ia, 1e-12f));
+            ca[k] = ecoef(P, k, fmaxs((float)U.m[k] * ia, 1e-12f));
+        }
+        const float p_before =
+            fmaxs(pressure_shifted<M>(P, e_hat_before - 0.5f * rho_before * v2b, alpha, cb), 1e-10f);
+        U.E = energy_shifted<M>(P, p_before, alpha, ca) + fix + 0.5f * rho * v2b;
     }
     const float inv = rcp(fmaxs(rho, 1e-10f));
     float v2 = -0.f;
@@ -454,19 +508,17 @@ __device__ __forceinline__ void reconstruct(const Pr<D, M>& L, const Pr<D, M>& C
     enforce<D, M>(rf);
 }
 
-// sum_k alpha_k / (rho_k c_k^2) with rho_k c_k^2 = gamma_k (p + p_inf_k) for
-// ideal and stiffened gases (the Wood / Kapila compressibility)
+// sum_k alpha_k / (rho_k c_k^2), the Wood compressibility (eos.hpp:195-221),
+// with rho_k c_k^2 = gamma_k (p' + p_inf_k) at the clamped partial density
+// (impedance; gamma_k (p + p_inf_k) unless the fraction is near-massless)
 template<int M>
-__device__ __forceinline__ float compressibility(const EosMixed& P, float p, const float* alpha, float* imp,
-                                                 bool* active, float floor_alpha)
+__device__ __forceinline__ float compressibility(const EosMixed& P, float p, const float* alpha, const float* rho_k)
 {
     float beta = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) {
         const float a = fmaxs(alpha[k], 0.f);
-        imp[k] = fmaxs(P.gamma[k] * (p + P.pinf[k]), 1e-12f);
-        active[k] = !(a <= floor_alpha);
-        if (active[k]) beta += a * rcp(imp[k]);
+        beta += a * rcp(fmaxs(impedance(P, k, p, ecoef(P, k, rho_k[k])), 1e-12f));
     }
     return beta;
 }
@@ -482,25 +534,26 @@ __device__ __forceinline__ float sound_speed(const EosMixed& P, float rho, float
         if constexpr (M == 2) {
             // c^2 = 1/(rho (a0/I0 + a1/I1)) = Y/(rho X), X = a0 I1 + a1 I0, Y = I0 I1,
             // so c = Y rsq(rho X Y): one transcendental
-            const float I0 = fmaxs(P.gamma[0] * (p + P.pinf[0]), 1e-12f);
-            const float I1 = fmaxs(P.gamma[1] * (p + P.pinf[1]), 1e-12f);
+            const float I0 = fmaxs(impedance(P, 0, p, ecoef(P, 0, rho_k[0])), 1e-12f);
+            const float I1 = fmaxs(impedance(P, 1, p, ecoef(P, 1, rho_k[1])), 1e-12f);
             const float a0 = fmaxs(alpha[0], 0.f), a1 = fmaxs(alpha[1], 0.f);
             const float X = fmaxs(a0 * I1 + a1 * I0, 1e-30f);
             const float Y = I0 * I1;
             return Y * frsq(rho * X * Y);
         } else {
-            float imp[M];
-            bool act[M];
-            const float beta = compressibility<M>(P, p, alpha, imp, act, -1.f);
+            const float beta = compressibility<M>(P, p, alpha, rho_k);
             return frsq(rho * fmaxs(beta, 1e-14f));
         }
     }
-    float cs[M], xi_mix = -0.f, inv_sum = -0.f;
+    // xi_k = rho_k de_k/dp = w_k/(gamma_k - 1) (eos.hpp:42-57 on the ideal and
+    // stiffened laws), c_k^2 = rho' c^2 / rho'
+    float cs[M], xi[M], xi_mix = -0.f, inv_sum = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) {
         const float a = fmaxs(alpha[k], 0.f);
-        cs[k] = fsqrt(P.gamma[k] * (p + P.pinf[k]) * rcp(fmaxs(rho_k[k], 1e-10f)));
-        xi_mix += a * P.inv_gm1[k];
+        xi[k] = ecoef(P, k, rho_k[k]);
+        cs[k] = fsqrt(impedance(P, k, p, xi[k]) * rcp(fmaxs(rho_k[k], 1e-10f)));
+        xi_mix += a * xi[k];
         inv_sum += a * rcp(fmaxs(cs[k], 1e-14f));
     }
     float rc2 = -0.f;
@@ -508,9 +561,17 @@ __device__ __forceinline__ float sound_speed(const EosMixed& P, float rho, float
     for (int k = 0; k < M; ++k) {
         const float a = fmaxs(alpha[k], 0.f);
         const float lam = (P.lambda_model == LAMBDA_EQUAL_VELOCITY) ? rcp(fmaxs(cs[k], 1e-14f) * inv_sum) : 1.f;
-        rc2 += lam * a * P.inv_gm1[k] * cs[k] * cs[k];
+        rc2 += lam * a * xi[k] * cs[k] * cs[k];
     }
     return fsqrt(fmaxs(rc2 * rcp(xi_mix) * rcp(rho), 1e-14f));
+}
+
+// acoustic_impedance of alpha_rhs_coefficients (primitives.hpp:288-297):
+// max(rho_k c_k^2, 1e-12) with rho_k the primitive's partial density (floored
+// at 1e-12) and c_k from the clamped one (1e-10), rho_k/rho' = min(1, 1e10 rho_k)
+__device__ __forceinline__ float kapila_impedance(const EosMixed& P, int k, float p, float rho_k)
+{
+    return fmaxs(fmins(rho_k * 1e10f, 1.f) * impedance(P, k, p, ecoef(P, k, rho_k)), 1e-12f);
 }
 
 // alpha_rhs_coefficients (primitives.hpp:216-322)
@@ -531,7 +592,7 @@ __device__ __forceinline__ void alpha_rhs(const EosMixed& P, const Pr<D, M>& Q, 
         for (int k = 0; k < M; ++k) {
             const float a = rhs[k];
             act[k] = !(a <= 1e-12f || a < srcf);
-            cs[k] = act[k] ? fsqrt(P.gamma[k] * (p + P.pinf[k]) * rcp(fmaxs(Q.rho[k], 1e-12f))) : 1e-12f;
+            cs[k] = act[k] ? fsqrt(impedance(P, k, p, ecoef(P, k, Q.rho[k])) * rcp(fmaxs(Q.rho[k], 1e-10f))) : 1e-12f;
             if (act[k]) inv_sum += a * rcp(cs[k]);
         }
         if (inv_sum <= 1e-12f || !ffinite(inv_sum)) return;
@@ -547,8 +608,8 @@ __device__ __forceinline__ void alpha_rhs(const EosMixed& P, const Pr<D, M>& Q, 
         // one reciprocal for the pair
         const float a0 = rhs[0], a1 = rhs[1];
         if (!(a0 <= 1e-12f || a0 < srcf) && !(a1 <= 1e-12f || a1 < srcf)) {
-            const float I0 = fmaxs(P.gamma[0] * (p + P.pinf[0]), 1e-12f);
-            const float I1 = fmaxs(P.gamma[1] * (p + P.pinf[1]), 1e-12f);
+            const float I0 = kapila_impedance(P, 0, p, Q.rho[0]);
+            const float I1 = kapila_impedance(P, 1, p, Q.rho[1]);
             const float invX = rcp(a0 * I1 + a1 * I0);
             rhs[0] = a0 + a0 * (I1 * invX - 1.f);
             rhs[1] = a1 + a1 * (I0 * invX - 1.f);
@@ -562,7 +623,7 @@ __device__ __forceinline__ void alpha_rhs(const EosMixed& P, const Pr<D, M>& Q, 
     for (int k = 0; k < M; ++k) {
         const float a = rhs[k];
         act[k] = !(a <= 1e-12f || a < srcf);
-        imp[k] = fmaxs(P.gamma[k] * (p + P.pinf[k]), 1e-12f);
+        imp[k] = kapila_impedance(P, k, p, Q.rho[k]);
         if (act[k]) beta += a * rcp(imp[k]);
     }
     if (beta <= 1e-12f || !ffinite(beta)) return;

@@ -267,28 +267,40 @@ def batch_instances(count=64, seed=20261203, nx_range=(256, 1024), ny_range=(256
         rng = np.random.default_rng(seed + i)
         nx = int(rng.integers(nx_range[0], nx_range[1] + 1))
         ny = int(rng.integers(ny_range[0], ny_range[1] + 1))
-        H = 1.0
-        Lx = nx / ny * H
         ga, gb = rng.uniform(1.3, 1.7, 2)
         ratio = rng.uniform(0.1, 10.0)
         mach = rng.uniform(1.1, 3.0)
-        r = rng.uniform(0.1, 0.3) * H
-        rho1, p1 = 1.0, 1.0e5
-        rho2, p2, u2 = rankine_hugoniot(ga, mach, rho1, p1)
-        mats = [("ideal_gas", float(ga), 0.0), ("ideal_gas", float(gb), 0.0)]
-        x0 = 0.4 * Lx
-        shock_x = min(x0 + r + 0.1 * Lx, 0.9 * Lx)
-        if i % 2 == 0:
-            shape = None
-        else:
-            amp = 0.5 * r
-            k = int(rng.integers(1, 4))
-            shape = (lambda X, x0=x0, amp=amp, k=k: X[0] - (x0 + amp * np.sin(2.0 * np.pi * k * X[1] / H)))
-        c = shock_bubble((nx, ny), (0.0, 0.0), (Lx, H), mats, 0, 1,
-                         pre=(rho1, (0.0, 0.0), p1), post=(rho2, (-u2, 0.0), p2),
-                         bubble=(rho1 * ratio, (0.0, 0.0), p1), centre=(x0, 0.5 * H), radius=r,
-                         shock_axis=0, shock_position=shock_x, thickness=2.0 * H / ny,
-                         bc_lo=("transmissive", "reflective"), bc_hi=("transmissive", "reflective"),
-                         name=f"batch_{i}_{nx}x{ny}", bubble_shape=shape)
-        cases.append(c)
+        r = rng.uniform(0.1, 0.3)
+        k = 0 if i % 2 == 0 else int(rng.integers(1, 4))
+        cases.append(shock_interface(nx, ny, float(ga), float(gb), ratio, mach, r, k, name=f"batch_{i}_{nx}x{ny}"))
     return cases
+
+
+def shock_interface(nx, ny, gamma_a, gamma_b, ratio, mach, radius, wavenumber=0, name=None):
+    """One shock-interface instance of the batched sweep on [0, nx/ny] x [0, 1]
+    (square cells): ambient ideal gas a (rho 1, p 1e5, at rest) with a
+    region of ideal gas b (density ratio*1, same pressure) whose interface is
+    a circle of the given radius centred at (0.4 L, 0.5) (wavenumber 0), or
+    the sinusoid x = 0.4 L + (radius/2) sin(2 pi k y) with k = wavenumber;
+    a Mach `mach` shock in gas a (Rankine-Hugoniot) at
+    x = min(0.4 L + radius + 0.1 L, 0.9 L) moving -x; transmissive in x,
+    reflective in y; interface thickness 2 cells, tanh sharpness 2."""
+    H = 1.0
+    Lx = nx / ny * H
+    r = radius * H
+    rho1, p1 = 1.0, 1.0e5
+    rho2, p2, u2 = rankine_hugoniot(gamma_a, mach, rho1, p1)
+    mats = [("ideal_gas", float(gamma_a), 0.0), ("ideal_gas", float(gamma_b), 0.0)]
+    x0 = 0.4 * Lx
+    shock_x = min(x0 + r + 0.1 * Lx, 0.9 * Lx)
+    if wavenumber == 0:
+        shape = None
+    else:
+        amp = 0.5 * r
+        shape = (lambda X, x0=x0, amp=amp, k=wavenumber: X[0] - (x0 + amp * np.sin(2.0 * np.pi * k * X[1] / H)))
+    return shock_bubble((nx, ny), (0.0, 0.0), (Lx, H), mats, 0, 1,
+                        pre=(rho1, (0.0, 0.0), p1), post=(rho2, (-u2, 0.0), p2),
+                        bubble=(rho1 * ratio, (0.0, 0.0), p1), centre=(x0, 0.5 * H), radius=r,
+                        shock_axis=0, shock_position=shock_x, thickness=2.0 * H / ny,
+                        bc_lo=("transmissive", "reflective"), bc_hi=("transmissive", "reflective"),
+                        name=name or f"shock_interface_{nx}x{ny}", bubble_shape=shape)
