@@ -18,6 +18,15 @@
 // instead of building primitive rows on the host; its tanh is the device
 // library's, so diffuse initial cells may differ from the reference's in the
 // last bit.
+//
+// Two environment variables reach what the case format cannot express:
+//   DIM_MAX_STEPS=<n>         stop after n steps of each resolution (before
+//                             tfinal if it comes first), then write the
+//                             end-of-run outputs as at tfinal; the reference
+//                             has no step-cap key (run_dim.hpp:88).
+//   DIMHIP_DUMP_INITIAL=<f>   write the packed FP64 initial state (ncells x
+//                             (2 nmat + dim) doubles, the reference's per-cell
+//                             order) of each resolution to <f>.<index>.
 #include <algorithm>
 #include <array>
 #include <chrono>
@@ -49,6 +58,17 @@ struct Options {
 inline void check(int status, const dimhip_ctx* ctx)
 {
     if (status != 0) throw std::runtime_error(dimhip_last_error(ctx));
+}
+
+// DIM_MAX_STEPS: a non-negative step count, or -1 when unset
+inline long long env_max_steps()
+{
+    const char* v = std::getenv("DIM_MAX_STEPS");
+    if (!v) return -1;
+    char* end = nullptr;
+    const long long n = std::strtoll(v, &end, 10);
+    if (end == v || *end != '\0' || n < 0) throw std::runtime_error("DIM_MAX_STEPS must be a non-negative integer");
+    return n;
 }
 
 inline bool env_flag(const char* name)
@@ -213,7 +233,7 @@ private:
 
 // run_dim_case (run_dim.hpp:18-186)
 template<int DIM>
-void run_case(const CaseConfig<DIM>& c, const std::array<int, DIM>& N, const Options& opt)
+void run_case(const CaseConfig<DIM>& c, const std::array<int, DIM>& N, const Options& opt, int n_index)
 {
     if (c.barton_solid_material >= 0)
         throw std::runtime_error(
@@ -231,6 +251,16 @@ void run_case(const CaseConfig<DIM>& c, const std::array<int, DIM>& N, const Opt
         const std::vector<double> prim = ic.rows();
         check(dimhip_upload_primitive(ctx.get(), prim.data(), (size_t)ncells), ctx.get());
     }
+    if (const char* dump = std::getenv("DIMHIP_DUMP_INITIAL")) {
+        std::vector<double> packed((size_t)ncells * (2 * nmat + DIM));
+        check(dimhip_download(ctx.get(), packed.data(), (size_t)ncells), ctx.get());
+        const std::string path = std::string(dump) + "." + std::to_string(n_index);
+        FILE* f = std::fopen(path.c_str(), "wb");
+        if (!f || std::fwrite(packed.data(), sizeof(double), packed.size(), f) != packed.size())
+            throw std::runtime_error("DIMHIP_DUMP_INITIAL: cannot write " + path);
+        std::fclose(f);
+    }
+    const long long max_steps = env_max_steps();
     const bool timing_only = std::getenv("QUANT_TIMING_ONLY") != nullptr;
     const bool track = env_flag("SOLVER_CONSERVATION");
     const int interval = env_interval();
@@ -276,20 +306,22 @@ void run_case(const CaseConfig<DIM>& c, const std::array<int, DIM>& N, const Opt
     while (next_output < c.output_times.size() && c.output_times[next_output] <= 1e-14)
         snapshot(time_tag(c.output_times[next_output++]));
 
-    while (time < c.tfinal - 1e-14) {
+    while (time < c.tfinal - 1e-14 && (max_steps < 0 || step < max_steps)) {
         const double target = next_output < c.output_times.size() ? c.output_times[next_output] : c.tfinal;
         // steps until the next output time (the device ends the run of steps there),
         // until the next conservation sample, or a bounded batch whose kernel
         // timings are then collected
         int chunk = use_graph ? 1 << 20 : 2048;
         if (track) chunk = std::min<long long>(chunk, interval - step % interval);
+        if (max_steps >= 0) chunk = (int)std::min<long long>(chunk, max_steps - step);
         long long taken = 0;
         check(dimhip_advance(ctx.get(), chunk, c.cfl, target, use_graph ? 1 : 0, &time, &taken), ctx.get());
         step += taken;
         if (taken == 0) throw std::runtime_error("run_dim_case: non-positive timestep");
         while (next_output < c.output_times.size() && time >= c.output_times[next_output] - 1e-14)
             snapshot(time_tag(c.output_times[next_output++]));
-        if (log && (step % interval == 0 || time >= c.tfinal - 1e-14)) {
+        const bool last = time >= c.tfinal - 1e-14 || (max_steps >= 0 && step >= max_steps);
+        if (log && (step % interval == 0 || last)) {
             const auto t0 = now();
             check(dimhip_boundary_flux_integral(ctx.get(), flux_integral.data()), ctx.get());
             log->write(step, time, totals(), flux_integral);
@@ -346,11 +378,12 @@ int main(int argc, char** argv)
         const CaseConfig<DIM_> c = load_case<DIM_>(opt.config);
         std::cout << "Regions parsed: " << c.regions.size() << "\n";
         std::cout << "Interface method: " << c.interface_method << "\n";
-        for (const auto& N : c.N_list) {
+        for (size_t n_index = 0; n_index < c.N_list.size(); ++n_index) {
+            const auto& N = c.N_list[n_index];
             std::cout << "Running N = ";
             for (int d = 0; d < DIM_; ++d) std::cout << N[d] << " ";
             std::cout << "\n";
-            if (c.interface_method == "DIM") run_case<DIM_>(c, N, opt);
+            if (c.interface_method == "DIM") run_case<DIM_>(c, N, opt, (int)n_index);
             else if (c.interface_method == "GFM" || c.interface_method == "SM")
                 throw std::runtime_error("interface_method " + c.interface_method +
                                          " runs on the reference's sharp-interface solver, not this DIM driver");

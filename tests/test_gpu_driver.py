@@ -16,7 +16,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from oracle.pyoracle import make_params, ref_available
+from oracle.pyoracle import make_params, ref_available, ref_initial_state
 from diffuse_vs_gfm_mms_amd import dimhip, workloads as W
 from diffuse_vs_gfm_mms_amd.build import driver_path
 
@@ -255,6 +255,100 @@ def test_device_initial_conditions_match_host(tmp_path, name):
         bad = np.argwhere(np.abs(A - B) > tol)
         detail = [(ha[j], int(i), A[i, j], B[i, j]) for i, j in bad[:8]]
         assert len(bad) == 0, (name, k, len(bad), detail)
+
+
+# case files whose interfaces are sharp (interface_thickness = 0): no tanh
+SHARP = {"water_air_1d_sharp.txt"}
+
+
+def dump_initial(tmp_path, name, extra=()):
+    """The driver's packed FP64 initial state (DIMHIP_DUMP_INITIAL) of the
+    first resolution of a case file, taking no step (DIM_MAX_STEPS=0)."""
+    dim = CASE_FILES[name]
+    out = tmp_path / "out"
+    case = write_case(tmp_path, name, out)
+    dump = tmp_path / "ic"
+    env = dict(os.environ, DIMHIP_DUMP_INITIAL=str(dump), DIM_MAX_STEPS="0")
+    print("running", driver_path(dim), case, *extra, flush=True)
+    res = subprocess.run([driver_path(dim), case, *extra], capture_output=True, text=True, env=env, timeout=300)
+    assert res.returncode == 0, res.stdout + res.stderr
+    return case, np.fromfile(str(dump) + ".0", dtype=np.float64)
+
+
+@pytest.mark.skipif(not ref_available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("name", sorted(CASE_FILES))
+def test_device_initial_state_matches_reference_fp64(tmp_path, name):
+    """dimhip_initialise (--device-ic) against the reference's own initial
+    state (initialise_dim_from_config, dim_initial_conditions.hpp:16-628,
+    from oracle/_ref), packed FP64: bit for bit where interfaces are sharp;
+    where they are tanh-blended the device tanh may differ from the host C
+    library's in its last bit, and every field stays within 1e-15 of its
+    scale (the largest |value| of the field) -- a bound the host-built rows
+    (dimhip_upload_primitive) meet with equality of bits."""
+    dim = CASE_FILES[name]
+    case, U = dump_initial(tmp_path, name, ["--device-ic"])
+    U_ref = ref_initial_state(case, dim, 0)
+    U = U.reshape(U_ref.shape)
+    host = tmp_path / "host"
+    host.mkdir()
+    _, U_host = dump_initial(host, name)
+    assert np.array_equal(U_host.reshape(U_ref.shape).view(np.uint64), U_ref.view(np.uint64))
+    same = np.all(U.view(np.uint64) == U_ref.view(np.uint64), axis=1)
+    scale = np.maximum(np.abs(U_ref).max(axis=0), 1e-300)
+    rel = (np.abs(U - U_ref) / scale).max(axis=0)
+    print(f"{name}: {int((~same).sum())} of {len(same)} cells differ in any bit; max |diff|/scale per field {rel}")
+    if name in SHARP:
+        assert same.all()
+    assert np.all(rel <= 1e-15), rel
+
+
+@pytest.mark.skipif(not ref_available(), reason="reference build (oracle/_ref) not present")
+def test_step_cap_matches_reference_run_to_the_same_time(tmp_path):
+    """DIM_MAX_STEPS=K (tfinal far beyond the K steps, no output times): the
+    driver stops after K steps and writes its end-of-run CSV.  The reference's
+    driver, which has no step cap, is run to the time of those K steps
+    (tfinal = t_K to 17 digits): same step count, and the same CSV values to
+    the six digits both print (its last step may be clamped by an ulp of t)."""
+    K = 37
+    name = "helium_bubble_2d.txt"
+    text = open(os.path.join(CASES, name)).read()
+    base = [l for l in text.splitlines() if not l.strip().startswith(("output_times", "tfinal"))]
+
+    def case_with(tag, tfinal):
+        p = tmp_path / f"{tag}.txt"
+        p.write_text("\n".join(base + [f"tfinal = {tfinal!r}", "output_times = []"]).replace(
+            "OUTDIR", str(tmp_path / tag)) + "\n")
+        return str(p)
+
+    # t_K: the FP64 path from the driver's initial state (the reference's bits)
+    c0 = case_with("probe", 1.0)
+    env = dict(os.environ, DIMHIP_DUMP_INITIAL=str(tmp_path / "ic"), DIM_MAX_STEPS="0")
+    res = subprocess.run([driver_path(2), c0], capture_output=True, text=True, env=env, timeout=300)
+    assert res.returncode == 0, res.stdout + res.stderr
+    U0 = np.fromfile(str(tmp_path / "ic") + ".0", dtype=np.float64).reshape(-1, 6)
+    mats = [("ideal_gas", 1.4, 0.0), ("ideal_gas", 1.648, 0.0)]
+    with dimhip.Solver(2, (128, 36), (0.325 / 128, 0.089 / 36), mats, ("transmissive", "reflective"),
+                       ("transmissive", "reflective")) as s:
+        s.upload(U0)
+        tK, n = s.advance(K, 0.8, 1.0, 0.0, use_graph=False)
+    assert n == K
+    ours = case_with("ours", 1.0)
+    env = dict(os.environ, DIM_MAX_STEPS=str(K))
+    res = subprocess.run([driver_path(2), ours], capture_output=True, text=True, env=env, timeout=300)
+    assert res.returncode == 0, res.stdout + res.stderr
+    run(REF_MAIN.format(2), case_with("ref", tK))
+    fa, fb = files_under(tmp_path / "ours"), files_under(tmp_path / "ref")
+    assert sorted(fa) == sorted(fb)
+    for k in fa:
+        if k.endswith("_runtime.txt"):
+            ra = dict(l.split(" = ", 1) for l in open(fa[k]).read().splitlines() if " = " in l)
+            rb = dict(l.split(" = ", 1) for l in open(fb[k]).read().splitlines() if " = " in l)
+            assert ra["steps"] == rb["steps"] == str(K)
+        elif k.endswith(".csv"):
+            ha, A = read_csv_numbers(fa[k])
+            hb, B = read_csv_numbers(fb[k])
+            assert ha == hb and A.shape == B.shape
+            assert np.allclose(A, B, rtol=2e-6, atol=1e-12 * np.abs(B).max()), k
 
 
 def test_device_initial_condition_matches_numpy_builder():
