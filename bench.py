@@ -141,6 +141,20 @@ def field_l1(a, b):
     return float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
 
 
+def mass_balance(case, c0, c1, flux_integral):
+    """Relative mass change and the reference's boundary-flux-corrected mass
+    balance (conservation_report.hpp:506-520, 596-607, 625-635) from the
+    device's FP64 totals at the start (c0) and end (c1) and its time integral
+    of the boundary flux; totals layout [mass_k (nmat), mass, mom_d, E]."""
+    k = case.nmat
+    m0, m1, flux = c0[k], c1[k], flux_integral[k]
+    scale = max(abs(m0), abs(m1), 1e-12)
+    residual = (m1 - m0) + flux
+    return {"relative_mass_change": (m1 - m0) / m0,
+            "balance": {"raw_drift": m1 - m0, "boundary_flux": flux, "residual": residual, "scale": scale,
+                        "normalized_residual": abs(residual) / scale}}
+
+
 def accuracy_vs_fp64(dimhip, case, steps, label):
     """The FP32-mixed form against the FP64 form (bit-identical to the
     reference's solver) at the same physical time: the FP64 run takes `steps`
@@ -157,23 +171,23 @@ def accuracy_vs_fp64(dimhip, case, steps, label):
     out = {}
     with dimhip.Solver.from_case(case, precision="fp64") as s:
         U0 = s.download()
-        m0 = s.conservation()[case.nmat]
+        s.track_boundary_flux(True)
+        c0 = s.conservation()
         T, n64 = s.advance(steps, case.cfl, 1.0, 0.0, use_graph=True)
-        m1 = s.conservation()[case.nmat]
         rows["fp64"] = s.output_rows()
-        out["fp64"] = {"steps": n64, "relative_mass_change": (m1 - m0) / m0}
+        out["fp64"] = dict(steps=n64, **mass_balance(case, c0, s.conservation(), s.boundary_flux_integral()))
     noisy = U0 * (1.0 + 1e-7 * np.random.default_rng(12345).standard_normal(U0.shape))
     for tag, prec, U in (("fp32_mixed", "fp32_mixed", U0), ("fp64_noise", "fp64", noisy)):
         with dimhip.Solver(case.dim, case.N, case.dx, case.materials, case.bc_lo, case.bc_hi,
                            precision=prec) as s:
             s.upload(U)
-            m0 = s.conservation()[case.nmat]
+            s.track_boundary_flux(True)
+            c0 = s.conservation()
             t, n = s.advance(4 * steps + 100, case.cfl, T, 0.0, use_graph=True)
-            m1 = s.conservation()[case.nmat]
             if abs(t - T) > 1e-12 * T:
                 raise RuntimeError(f"{tag} run ended at t={t}, not at {T}")
             rows[tag] = s.output_rows()
-            out[tag] = {"steps": n, "relative_mass_change": (m1 - m0) / m0}
+            out[tag] = dict(steps=n, **mass_balance(case, c0, s.conservation(), s.boundary_flux_integral()))
         del U
     D = case.dim
     b = rows["fp64"]
@@ -195,6 +209,16 @@ def accuracy_vs_fp64(dimhip, case, steps, label):
         "relative_mass_change_fp64": out["fp64"]["relative_mass_change"],
         "relative_mass_change_fp32_mixed": out["fp32_mixed"]["relative_mass_change"],
         "mass_change_difference": out["fp32_mixed"]["relative_mass_change"] - out["fp64"]["relative_mass_change"],
+        "mass_balance": {
+            "definition": "(M(T) - M(0)) + integral of the boundary mass flux of each step's new state, dt-weighted "
+                          "(accumulate_fluxes, conservation_report.hpp:506-520), normalised by max(|M(0)|, "
+                          "max |M|, 1e-12) (:596-607, 625-635)",
+            "fp64": out["fp64"]["balance"],
+            "fp32_mixed": out["fp32_mixed"]["balance"],
+            "fp64_noise_1e-7": out["fp64_noise"]["balance"],
+            "normalized_residual_difference": out["fp32_mixed"]["balance"]["residual"] / out["fp64"]["balance"]["scale"]
+            - out["fp64"]["balance"]["residual"] / out["fp64"]["balance"]["scale"],
+        },
     }
 
 

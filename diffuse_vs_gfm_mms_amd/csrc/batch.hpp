@@ -113,7 +113,7 @@ batch_begin_step_kernel(const BatchInst<Pol>* __restrict__ insts, int n, int res
     for (int off = 32; off > 0; off >>= 1) m = smin(m, __shfl_xor(m, off, 64));
     if (k != 0) return;
     ctl->dtmin[s] = double_to_bits(m);
-    int active = ctl->time < ctl->t_end - 1e-14;
+    int active = ctl->time < ctl->t_end - 1e-14 && !ctl->err;
     const double dt_max = ctl->t_end - ctl->time;
     const double dt = smin(dt_max, m);
     if (active && !(dt > 0.0)) {

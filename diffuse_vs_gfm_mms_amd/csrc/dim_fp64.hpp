@@ -749,14 +749,19 @@ __device__ __forceinline__ void hancock_predict(const EP& P, const State<D, M>& 
 // Returns false on the reference's throwing branch (non-finite density or
 // pressure in require_positive, numerical_safety.hpp:67-78).
 template<int D, int M, int DIR, typename EP>
-__device__ __forceinline__ bool hllc_flux(const EP& P, const State<D, M>& UL,
+__device__ __forceinline__ int hllc_flux(const EP& P, const State<D, M>& UL,
                                           const State<D, M>& UR, const Prim<D, M>& PL,
                                           const Prim<D, M>& PR, Flux<D, M>& F, double& fv)
 {
     double rhoL = total_density<D, M>(UL);
     double rhoR = total_density<D, M>(UR);
     double pL = PL.p, pR = PR.p;
-    if (!finite(rhoL) || !finite(rhoR) || !finite(pL) || !finite(pR)) return false;
+    // require_positive on each input in the reference's order: a non-finite
+    // one fails the face (status = its error bit), a non-positive one is raised
+    if (!finite(rhoL)) return ERR_HLLC_LEFT_DENSITY;
+    if (!finite(rhoR)) return ERR_HLLC_RIGHT_DENSITY;
+    if (!finite(pL)) return ERR_HLLC_LEFT_PRESSURE;
+    if (!finite(pR)) return ERR_HLLC_RIGHT_PRESSURE;
     if (rhoL <= 0.0) rhoL = 1e-10;
     if (rhoR <= 0.0) rhoR = 1e-10;
     if (pL <= 0.0) pL = 1e-10;
@@ -784,17 +789,17 @@ __device__ __forceinline__ bool hllc_flux(const EP& P, const State<D, M>& UL,
         for (int d = 0; d < D; ++d) F.mom[d] = 0.5 * (FL.mom[d] + FR.mom[d]);
         F.E = 0.5 * (FL.E + FR.E);
         fv = 0.5 * (unL + unR);
-        return true;
+        return 0;
     }
     if (0.0 <= sL) {
         F = FL;
         fv = unL;
-        return true;
+        return 0;
     }
     if (sR <= 0.0) {
         F = FR;
         fv = unR;
-        return true;
+        return 0;
     }
     const bool left = (0.0 <= s_star);  // sL < 0 here
     const State<D, M>& UK = left ? UL : UR;
@@ -818,7 +823,7 @@ __device__ __forceinline__ bool hllc_flux(const EP& P, const State<D, M>& UL,
         for (int d = 0; d < D; ++d)
             F.mom[d] = inv * ((sR * FL.mom[d] - sL * FR.mom[d]) + sLsR * (UR.mom[d] - UL.mom[d]));
         F.E = inv * ((sR * FL.E - sL * FR.E) + sLsR * (UR.E - UL.E));
-        return true;
+        return 0;
     }
     const double mass_scale = safe_div(rho_star, rhoK);
     const double delta = s_star - unK;
@@ -831,7 +836,7 @@ __device__ __forceinline__ bool hllc_flux(const EP& P, const State<D, M>& UL,
         F.mom[d] = FK.mom[d] + sK * (rho_star * u_star - UK.mom[d]);
     }
     F.E = FK.E + sK * (E_star - UK.E);
-    return true;
+    return 0;
 }
 
 }  // namespace dimhip

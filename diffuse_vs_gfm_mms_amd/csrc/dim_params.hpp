@@ -71,9 +71,31 @@ struct StepCtl {
     int active;
     int use_t_end;
     int err;
+    // the first failing HLLC face in the reference's serial order (the one
+    // whose require_positive would throw first): the smallest hllc_err_key
+    unsigned long long err_key;
 };
 
-enum : int { ERR_NONFINITE_HLLC = 1, ERR_NONPOSITIVE_DT = 2 };
+// Device error bits.  The HLLC ones follow the reference's require_positive
+// checks in their order (hllc.hpp:88-91): a face reports the first of them
+// its inputs fail, and the host names the first bit set in that order.
+enum : int {
+    ERR_HLLC_LEFT_DENSITY = 1,
+    ERR_NONPOSITIVE_DT = 2,
+    ERR_HLLC_RIGHT_DENSITY = 4,
+    ERR_HLLC_LEFT_PRESSURE = 8,
+    ERR_HLLC_RIGHT_PRESSURE = 16
+};
+
+// Order key of a failing HLLC face: sweep direction, then the reference's
+// line index (sweep_core.hpp:173-185), then the face along the line
+// (sweep_core.hpp:84-92), then which input check failed (hllc.hpp:88-91).
+__device__ __forceinline__ unsigned long long hllc_err_key(int dir, long long line, int face, int err_bit)
+{
+    const unsigned check = err_bit == ERR_HLLC_LEFT_DENSITY ? 0u : err_bit == ERR_HLLC_RIGHT_DENSITY ? 1u
+                           : err_bit == ERR_HLLC_LEFT_PRESSURE ? 2u : 3u;
+    return ((unsigned long long)dir << 59) | ((unsigned long long)line << 22) | ((unsigned long long)face << 2) | check;
+}
 
 // The CFL minimum of a launch is collected over kDtSlots addresses one
 // 128-byte line apart (block b writes slot b % kDtSlots), so that thousands

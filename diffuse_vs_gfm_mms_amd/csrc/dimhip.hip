@@ -237,6 +237,7 @@ __global__ void init_ctl_kernel(StepCtl* ctl)
         ctl->active = 1;
         ctl->use_t_end = 0;
         ctl->err = 0;
+        ctl->err_key = ~0ull;
     }
 }
 
@@ -247,6 +248,7 @@ __global__ void set_mode_kernel(StepCtl* ctl, int use_t_end, double t_end, doubl
         ctl->t_end = t_end;
         if (set_time) ctl->time = time;
         ctl->err = 0;
+        ctl->err_key = ~0ull;
     }
 }
 
@@ -332,10 +334,29 @@ void enqueue_step(dimhip_ctx* c, double cfl, double dt_max_host)
     if (c->track_flux) c->ops->boundary_flux(c, state_ptr(c, c->cur), c->flux_integral);
 }
 
+// The reference's message for a device error word.  A failing HLLC face
+// throws from its first failing require_positive (hllc.hpp:88-91, the message
+// and " (non-finite)", numerical_safety.hpp:67-78); of all failing faces the
+// one the reference meets first is the one with the smallest order key
+// (hllc_err_key), whose low two bits name the check.  Otherwise the caller's
+// dt check (run_dim.hpp:105-107).
+const char* device_error_text(int err, unsigned long long key)
+{
+    static const char* const hllc[4] = {
+        "dim::hllc_flux_normal: invalid left density (non-finite)",
+        "dim::hllc_flux_normal: invalid right density (non-finite)",
+        "dim::hllc_flux_normal: invalid left pressure (non-finite)",
+        "dim::hllc_flux_normal: invalid right pressure (non-finite)",
+    };
+    if (err & (ERR_HLLC_LEFT_DENSITY | ERR_HLLC_RIGHT_DENSITY | ERR_HLLC_LEFT_PRESSURE | ERR_HLLC_RIGHT_PRESSURE))
+        return hllc[key & 3u];
+    if (err & ERR_NONPOSITIVE_DT) return "run_dim_case: non-positive timestep";
+    return nullptr;
+}
+
 int check_device_error(dimhip_ctx* c)
 {
-    if (c->ctl_host->err & ERR_NONFINITE_HLLC) return fail(c, "dim::hllc_flux_normal: invalid left density (non-finite)");
-    if (c->ctl_host->err & ERR_NONPOSITIVE_DT) return fail(c, "run_dim_case: non-positive timestep");
+    if (const char* msg = device_error_text(c->ctl_host->err, c->ctl_host->err_key)) return fail(c, msg);
     return 0;
 }
 
@@ -948,11 +969,13 @@ int dimhip_kat(const dimhip_desc* desc, int which, int dir, int count, const dou
     };
     auto alloc = [&](double** p, size_t n) { return n ? hipMalloc(p, n * sizeof(double)) : hipSuccess; };
     if (alloc(&di0, n_in0) != hipSuccess || alloc(&di1, n_in1) != hipSuccess || alloc(&do0, n_out0) != hipSuccess ||
-        alloc(&do1, n_out1) != hipSuccess || hipMalloc(&derr, sizeof(int)) != hipSuccess) {
+        alloc(&do1, n_out1) != hipSuccess || hipMalloc(&derr, 4 * sizeof(int)) != hipSuccess) {
         cleanup();
         return fail(nullptr, "dimhip_kat: device allocation failed");
     }
-    hipMemset(derr, 0, sizeof(int));
+    // derr[0]: error bits; derr[2..3]: the smallest order key of a failing item
+    const int herr0[4] = {0, 0, -1, -1};
+    hipMemcpy(derr, herr0, sizeof(herr0), hipMemcpyHostToDevice);
     hipMemcpy(di0, in0, n_in0 * sizeof(double), hipMemcpyHostToDevice);
     if (n_in1) hipMemcpy(di1, in1, n_in1 * sizeof(double), hipMemcpyHostToDevice);
     int rc = 0;
@@ -966,15 +989,19 @@ int dimhip_kat(const dimhip_desc* desc, int which, int dir, int count, const dou
     }
     hipError_t e = hipDeviceSynchronize();
     int herr = 0;
+    unsigned long long hkey = ~0ull;
     if (e == hipSuccess) {
         hipMemcpy(out0, do0, n_out0 * sizeof(double), hipMemcpyDeviceToHost);
         if (n_out1) hipMemcpy(out1, do1, n_out1 * sizeof(double), hipMemcpyDeviceToHost);
-        hipMemcpy(&herr, derr, sizeof(int), hipMemcpyDeviceToHost);
+        int h[4];
+        hipMemcpy(h, derr, sizeof(h), hipMemcpyDeviceToHost);
+        herr = h[0];
+        std::memcpy(&hkey, h + 2, sizeof(hkey));
     }
     cleanup();
     if (e != hipSuccess) return fail(nullptr, std::string("HIP error: ") + hipGetErrorString(e));
     if (rc) return fail(nullptr, "dimhip_kat: launch failed");
-    if (herr) return fail(nullptr, "dim::hllc_flux_normal: invalid left density (non-finite)");
+    if (herr) return fail(nullptr, device_error_text(herr, hkey) ? device_error_text(herr, hkey) : "dimhip_kat: device error");
     return 0;
 }
 

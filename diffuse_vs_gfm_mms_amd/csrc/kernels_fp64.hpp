@@ -34,6 +34,9 @@ static __global__ void __launch_bounds__(64) begin_step_kernel(StepCtl* ctl, dou
         active = ctl->time < ctl->t_end - 1e-14;
         dt_max = ctl->t_end - ctl->time;
     }
+    // a step that raised an error ends the run, as the reference's exception
+    // does: later steps pass the state through
+    if (ctl->err) active = 0;
     const double dt = smin(dt_max, m);
     if (active && !(dt > 0.0)) {
         ctl->err |= ERR_NONPOSITIVE_DT;
@@ -160,7 +163,11 @@ __global__ void kat64_kernel(EP P, int which, int count, const double* __restric
         cons_to_prim<D, M>(P, UR, PR);
         Flux<D, M> F;
         double fv = 0.0;
-        if (!hllc_flux<D, M, DIR>(P, UL, UR, PL, PR, F, fv)) atomicOr(err, 1);
+        if (const int e = hllc_flux<D, M, DIR>(P, UL, UR, PL, PR, F, fv)) {
+            atomicOr(err, e);
+            // err[1..2]: the smallest (item, check) key, as a 64-bit value
+            atomicMin((unsigned long long*)(err + 2), hllc_err_key(0, 0, t, e));
+        }
         double* o = out0 + (long long)t * LY::NF;
         int j = 0;
 #pragma unroll

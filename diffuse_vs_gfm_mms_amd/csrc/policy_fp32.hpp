@@ -694,7 +694,7 @@ __device__ __forceinline__ void hancock(const EosMixed& P, const St<D, M>& Uc, c
 
 // hllc.hpp:62-195
 template<int D, int M, int DIR>
-__device__ __forceinline__ bool hllc(const EosMixed& P, const St<D, M>& UL, const St<D, M>& UR, const Pr<D, M>& PL,
+__device__ __forceinline__ int hllc(const EosMixed& P, const St<D, M>& UL, const St<D, M>& UR, const Pr<D, M>& PL,
                                      const Pr<D, M>& PR, Fx<D, M>& F, float& fv)
 {
     float rhoL = -0.f, rhoR = -0.f;
@@ -703,7 +703,10 @@ __device__ __forceinline__ bool hllc(const EosMixed& P, const St<D, M>& UL, cons
         rhoL += UL.m[k];
         rhoR += UR.m[k];
     }
-    if (!ffinite(rhoL) || !ffinite(rhoR) || !ffinite(PL.p) || !ffinite(PR.p)) return false;
+    if (!ffinite(rhoL)) return ERR_HLLC_LEFT_DENSITY;
+    if (!ffinite(rhoR)) return ERR_HLLC_RIGHT_DENSITY;
+    if (!ffinite(PL.p)) return ERR_HLLC_LEFT_PRESSURE;
+    if (!ffinite(PR.p)) return ERR_HLLC_RIGHT_PRESSURE;
     rhoL = rhoL <= 0.f ? 1e-10f : rhoL;
     rhoR = rhoR <= 0.f ? 1e-10f : rhoR;
     const float pL = PL.p <= 0.f ? 1e-10f : PL.p;
@@ -729,17 +732,17 @@ __device__ __forceinline__ bool hllc(const EosMixed& P, const St<D, M>& UL, cons
         for (int d = 0; d < D; ++d) F.mom[d] = 0.5f * (FL.mom[d] + FR.mom[d]);
         F.E = 0.5f * (FL.E + FR.E);
         fv = 0.5f * (unL + unR);
-        return true;
+        return 0;
     }
     if (0.f <= sL) {
         F = FL;
         fv = unL;
-        return true;
+        return 0;
     }
     if (sR <= 0.f) {
         F = FR;
         fv = unR;
-        return true;
+        return 0;
     }
     const bool left = (0.f <= s_star);
     const St<D, M>& UK = left ? UL : UR;
@@ -782,7 +785,7 @@ __device__ __forceinline__ bool hllc(const EosMixed& P, const St<D, M>& UL, cons
         for (int d = 0; d < D; ++d)
             F.mom[d] = inv * ((sR * FL.mom[d] - sL * FR.mom[d]) + sLsR * (UR.mom[d] - UL.mom[d]));
         F.E = inv * ((sR * FL.E - sL * FR.E) + sLsR * ((UR.E - UL.E) + dB));
-        return true;
+        return 0;
     }
 #pragma unroll
     for (int k = 0; k < M; ++k) F.m[k] = FK.m[k] + sK * (UK.m[k] * delta);
@@ -792,7 +795,7 @@ __device__ __forceinline__ bool hllc(const EosMixed& P, const St<D, M>& UL, cons
         F.mom[d] = FK.mom[d] + sK * (rho_star * u_star - UK.mom[d]);
     }
     F.E = FK.E + sK * (EK * delta + e_star_term);
-    return true;
+    return 0;
 }
 
 }  // namespace f32
@@ -1029,7 +1032,7 @@ struct Fp32Mixed {
         f32::hancock<D, M, DIR>(P, Uc, Pc, ul, ur, lf, rf, rhs, (float)dt, (float)dx, out);
     }
     template<int D, int M, int DIR>
-    static __device__ __forceinline__ bool hllc(const Eos& P, const St<D, M>& UL, const St<D, M>& UR,
+    static __device__ __forceinline__ int hllc(const Eos& P, const St<D, M>& UL, const St<D, M>& UR,
                                                 const Pr<D, M>& PL, const Pr<D, M>& PR, Fx<D, M>& F, float& fv)
     {
         return f32::hllc<D, M, DIR>(P, UL, UR, PL, PR, F, fv);

@@ -187,7 +187,7 @@ struct Fp64T {
         hancock_predict<D, M, DIR>(P, Uc, Pc, ul, ur, lf, rf, rhs, dt, dx, out);
     }
     template<int D, int M, int DIR>
-    static __device__ __forceinline__ bool hllc(const Eos& P, const St<D, M>& UL, const St<D, M>& UR,
+    static __device__ __forceinline__ int hllc(const Eos& P, const St<D, M>& UL, const St<D, M>& UR,
                                                 const Pr<D, M>& PL, const Pr<D, M>& PR, Fx<D, M>& F, double& fv)
     {
         return hllc_flux<D, M, DIR>(P, UL, UR, PL, PR, F, fv);

@@ -390,7 +390,15 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
         Pr PR;
         get_state<R, D, M>(sh, S, slot_r, 0, UR);
         get_prim<R, D, M>(sh, S, slot_r, NV, PR);
-        if (!Pol::template hllc<D, M, DIR>(P, Urh, UR, PUr, PR, F, fv)) atomicOr(&ctl->err, ERR_NONFINITE_HLLC);
+        if (const int e = Pol::template hllc<D, M, DIR>(P, Urh, UR, PUr, PR, F, fv)) {
+            // the reference's line index: the other axes ascending, last fastest
+            long long line = 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+                if (d != DIR) line = line * g.N[d] + idx[d];
+            atomicOr(&ctl->err, e);
+            atomicMin(&ctl->err_key, hllc_err_key(DIR, line, i, e));
+        }
         const R* up = (fv >= R(0)) ? PUr.alpha : PR.alpha;
 #pragma unroll
         for (int k = 0; k < M; ++k) af[k] = up[k] * fv;

@@ -77,6 +77,43 @@ def test_closed_box_mass_drift():
     assert np.max(np.abs(ring[:, 2:4])) < 1e-6
 
 
+def balance(s, c0, k):
+    """The reference's boundary-flux-corrected mass balance
+    (conservation_report.hpp:506-520, 596-607, 625-635): residual =
+    (M(T) - M(0)) + integral of dt * boundary mass flux of each new state,
+    normalised by max(|M(0)|, max |M|, 1e-12)."""
+    c1 = s.conservation()
+    res = (c1[k] - c0[k]) + s.boundary_flux_integral()[k]
+    return abs(res) / max(abs(c0[k]), abs(c1[k]), 1e-12), c1[k] - c0[k]
+
+
+def test_open_boundary_mass_balance():
+    """Helium shock-bubble 512x256, 200 steps (the post-shock gas flows in
+    through the right boundary): with the boundary flux integrated on the
+    device, the FP64 path's mass balance closes to roundoff (<= 1e-12), as the
+    reference's does.  The FP32-mixed form's residual is the FP32 rounding of
+    that inflow flux (its FP32 HLLC flux against the report's FP64 physical
+    flux of the same state, ~1e-7 relative): <= 5e-9, and the same order as
+    the difference of the two forms' mass changes."""
+    c = W.helium_shock_bubble_2d(512, 256)
+    k = c.nmat
+    out = {}
+    T = 1.0
+    for prec, n in (("fp64", 200), ("fp32_mixed", 1000)):
+        with dimhip.Solver.from_case(c, precision=prec) as s:
+            s.track_boundary_flux(True)
+            c0 = s.conservation()
+            # the FP32-mixed run ends at the FP64 run's time (its last step clamped)
+            T, _ = s.advance(n, c.cfl, T, 0.0)
+            out[prec] = balance(s, c0, k) + (c0[k],)
+    (r64, d64, m0), (r32, d32, _) = out["fp64"], out["fp32_mixed"]
+    print(f"normalised mass balance residual: fp64 {r64:.2e}, fp32-mixed {r32:.2e}; "
+          f"mass change difference {(d32 - d64) / m0:.2e}")
+    assert r64 <= 1e-12
+    assert r32 <= 5e-9
+    assert abs(d32 - d64) / m0 <= 5e-9
+
+
 def test_water_air_tube_fp32_mixed(oracle):
     """Stiffened-gas water/air tube (1000 cells, 300 steps).  The FP32-mixed
     form is accurate for gas-gas mixtures; in nearly pure water expanding at

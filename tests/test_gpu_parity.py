@@ -328,3 +328,43 @@ def test_pressure_bisection_matches_oracle(oracle, ms):
     got, _ = dimhip.kat(dimhip.KAT_PRESSURE, 1, mats, n, packed)
     ref = oracle.pressure(p, rt, e, alpha, rho)
     assert bits_equal(got, ref), mismatch_report(got, ref)
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_hllc_error_text_matches_oracle(oracle, case):
+    """A face whose density sum overflows fails the device HLLC with the
+    reference's message for its first failing check, and of several failing
+    faces the first in order names the error (hllc.hpp:88-91); the oracle's
+    messages are the reference's (tests/test_oracle.py)."""
+    from test_oracle import hllc_error_cases
+    name, UL, UR = hllc_error_cases()[case]
+    mats = [W.AIR, W.HELIUM]
+    with pytest.raises(RuntimeError) as e:
+        oracle.hllc(make_params(2, mats), 0, UL, UR)
+    with pytest.raises(RuntimeError) as d:
+        dimhip.kat(dimhip.KAT_HLLC, 2, mats, UL.shape[0], UL.reshape(-1), UR.reshape(-1), d=0)
+    assert str(d.value) == str(e.value), name
+
+
+@pytest.mark.parametrize("cells", [(0,), (63,), (0, 63)])
+def test_step_error_text_matches_oracle(oracle, cells):
+    """A 1D step over a line whose end cells' masses overflow the density sum
+    (an end cell's unpredicted state is its HLLC input): the device step fails
+    with the oracle's message -- the first failing face of the line, its
+    first failing check, as the reference's exception (whose what() the
+    reference's own OpenMP line loop ends the program with) -- and the
+    context keeps working after a fresh upload."""
+    c = W.water_air_tube_1d(64)
+    U = c.U.copy()
+    for j in cells:
+        U[j, :2] = 1.5e308
+    p = make_params(1, c.materials)
+    with pytest.raises(RuntimeError) as e:
+        oracle.advance(p, c.N, c.dx, c.bc_lo, c.bc_hi, U, c.cfl, 1.0)
+    with dimhip.Solver.from_case(c) as s:
+        s.upload(U)
+        with pytest.raises(RuntimeError) as d:
+            s.step(c.cfl, 1.0)
+        assert str(d.value) == str(e.value)
+        s.upload(c.U)
+        s.step(c.cfl, 1.0)

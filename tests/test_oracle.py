@@ -232,3 +232,36 @@ def test_minmod_planted_mistake_is_caught(oracle):
     second = oracle.solve_line(p, 0, U, 1e-3, 1e-8)
     # first order: the same face states as the cell centres, via the n==2 branch per face
     assert not np.allclose(second, U)
+
+
+def hllc_error_cases():
+    """(name, UL, UR) with one or two faces whose density sum overflows to
+    infinity: the only way total_density (a sum of finite_or'd masses,
+    primitives.hpp:99-108) becomes non-finite; cons_to_prim's pressure is
+    clamped finite by construction, so the pressure checks cannot fail."""
+    ok = np.array([1.0, 0.5, 10.0, 0.0, 2.5e5, 0.6])
+    big = ok.copy()
+    big[:2] = 1.5e308
+    return [
+        ("left", big[None], ok[None]),
+        ("right", ok[None], big[None]),
+        ("both", big[None], big[None]),
+        ("second face left, first face right", np.stack([ok, big]), np.stack([big, ok])),
+    ]
+
+
+@pytest.mark.skipif(not ref_available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("case", range(4))
+def test_hllc_error_text_matches_reference(oracle, case):
+    """The oracle fails on the same face with the same message as the
+    reference's hllc_flux_normal (hllc.hpp:88-91): the first failing face in
+    order, its first failing require_positive."""
+    name, UL, UR = hllc_error_cases()[case]
+    p = make_params(2, [W.AIR, W.HELIUM])
+    msgs = []
+    for kind in ("oracle", "ref"):
+        with pytest.raises(RuntimeError) as e:
+            Checker(kind).hllc(p, 0, UL, UR)
+        msgs.append(str(e.value))
+    assert msgs[0] == msgs[1], (name, msgs)
+    assert msgs[0].startswith("dim::hllc_flux_normal: invalid ") and msgs[0].endswith(" (non-finite)")
