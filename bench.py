@@ -49,11 +49,19 @@ def parse():
     ap.add_argument("--workload", default="helium_fp64", choices=sorted(WORKLOADS))
     ap.add_argument("--fp32-steps", type=int, default=2000,
                     help="steps of the FP32-mixed 4096x2048 companion measurement (0: skip)")
+    ap.add_argument("--fp64-c3-steps", type=int, default=200,
+                    help="steps of the FP64 form on the FP32-mixed grid (4096x2048), for the speed-up on one grid "
+                         "(0: skip)")
     ap.add_argument("--batch-steps", type=int, default=1000,
                     help="steps of the batched 64-instance companion measurement (0: skip)")
     ap.add_argument("--batch-count", type=int, default=64)
-    ap.add_argument("--cpu-steps", type=int, default=8,
-                    help="steps of the reference CPU solver timed on the host (about 10-30 s)")
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="steps per run of the reference CPU solver on helium 2048x1024 (0: the sample's default, "
+                         "2; 1 warm-up + 3 timed runs, about 15 s)")
+    ap.add_argument("--cpu-all", action="store_true",
+                    help="also time the reference CPU solver on the 4096x2048 grid, the 64 batched instances and "
+                         "the 3D bubble collapse at 128^3 (about 2 minutes)")
+    ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--accuracy-steps", type=int, default=-1,
                     help="steps of the FP32-mixed vs FP64 comparison at 4096x2048 (-1: as --fp32-steps, 0: skip "
                          "every FP32-mixed vs FP64 comparison)")
@@ -77,33 +85,129 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_for(workload):
-    """HBM bytes per sweep launch from the committed rocprofv3 --pmc summary, if any."""
+def traffic_for(workload, axis):
+    """HBM bytes per launch of the sweep along `axis` from the committed
+    rocprofv3 --pmc summary, if any."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_sweep_launch")
+        return json.load(f).get(f"{axis}_sweep_bytes")
 
 
-def cpu_baseline(case, nsteps):
-    """The reference's OpenMP solver on this host's cores (bounded sample)."""
-    from oracle.pyoracle import Checker, make_params, ref_available
+def valu_for(workload):
+    """The SQ-counter instruction mix of the workload's sweeps from the
+    committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"valu_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+# the reference's benchmark environment (tools/quant/runner.hpp:323-333)
+CPU_OMP_ENV = {"OMP_PROC_BIND": "close", "OMP_PLACES": "cores", "OMP_DYNAMIC": "FALSE", "OMP_SCHEDULE": "guided,1"}
+
+CPU_SAMPLES = {
+    # name: (description, steps per run)
+    "helium_2048x1024": ("FP64 helium shock-bubble 2048x1024", 2),
+    "helium_4096x2048": ("FP64 helium shock-bubble 4096x2048 (the FP32-mixed grid)", 1),
+    "batch_64": ("the 64 batched instances, FP64, one after another", 1),
+    "bubble_collapse_128": ("FP64 3D bubble collapse 128^3", 1),
+}
+
+
+def cpu_cases(name):
+    from diffuse_vs_gfm_mms_amd import workloads
+    if name == "helium_2048x1024":
+        return [workloads.helium_shock_bubble_2d(2048, 1024)]
+    if name == "helium_4096x2048":
+        return [workloads.helium_shock_bubble_2d(4096, 2048)]
+    if name == "batch_64":
+        return workloads.batch_instances(count=64)
+    if name == "bubble_collapse_128":
+        return [workloads.bubble_collapse_3d(128)]
+    raise ValueError(name)
+
+
+def physical_cores():
+    """Physical cores among the CPUs this process may run on (lscpu's core
+    ids of the affinity mask), the reference's "all physical cores"."""
+    import subprocess
+    cpus = sorted(os.sched_getaffinity(0))
+    try:
+        text = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, timeout=20).stdout
+        core_of = {}
+        for line in text.splitlines():
+            if line and not line.startswith("#"):
+                cpu, core, sock = line.split(",")[:3]
+                core_of[int(cpu)] = (sock, core)
+        cores = {core_of[c] for c in cpus if c in core_of}
+        if cores:
+            return len(cores), len(cpus)
+    except (OSError, ValueError):
+        pass
+    return len(cpus), len(cpus)
+
+
+def cpu_worker(spec):
+    """Child process: the reference's solver (oracle/_ref/libdimref.so, built
+    from its sources) on the sample's cases, `warmup` untimed passes then
+    `repeats` timed ones, each pass from the initial states; prints JSON."""
+    from oracle.pyoracle import Checker, make_params
+    cases = cpu_cases(spec["name"])
+    ref = Checker("ref")
+
+    def one_pass():
+        cells = secs = 0.0
+        for c in cases:
+            p = make_params(c.dim, c.materials)
+            _, _, t = ref.advance(p, c.N, c.dx, c.bc_lo, c.bc_hi, c.U, c.cfl, 1.0, nsteps=spec["steps"],
+                                  threads=spec["threads"])
+            cells += c.ncells * spec["steps"]
+            secs += t
+        return cells, secs
+
+    for _ in range(spec["warmup"]):
+        one_pass()
+    runs = [one_pass() for _ in range(spec["repeats"])]
+    print(json.dumps({"cell_updates": runs[0][0], "seconds": [r[1] for r in runs]}))
+
+
+def cpu_baseline(name, steps=None, repeats=3, warmup=1):
+    """The reference's OpenMP solver on this host: all physical cores of the
+    affinity mask, the reference's OpenMP environment, 1 warm-up pass and the
+    median of 3 (scripts/run_report2_local.sh:13-14), cells x steps / time of
+    the time loop (report_builders.hpp:298-300), in a child process so the
+    environment reaches the OpenMP runtime."""
+    import statistics
+    import subprocess
+    from oracle.pyoracle import ref_available
 
     if not ref_available():
         return None
-    threads = min(16, os.cpu_count() or 1)
-    ref = Checker("ref")
-    p = make_params(case.dim, case.materials)
-    _, _, secs = ref.advance(p, case.N, case.dx, case.bc_lo, case.bc_hi, case.U, case.cfl, 1.0,
-                             nsteps=nsteps, threads=threads)
+    desc, default_steps = CPU_SAMPLES[name]
+    steps = steps or default_steps
+    cores, cpus = physical_cores()
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), **CPU_OMP_ENV)
+    spec = {"name": name, "steps": steps, "repeats": repeats, "warmup": warmup, "threads": cores}
+    res = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker", json.dumps(spec)],
+                         capture_output=True, text=True, env=env, timeout=3000)
+    if res.returncode != 0:
+        raise RuntimeError("CPU baseline worker failed: " + res.stderr[-2000:])
+    r = json.loads(res.stdout.strip().splitlines()[-1])
+    rates = [r["cell_updates"] / t for t in r["seconds"]]
     return {
-        "value": case.ncells * nsteps / secs,
+        "value": statistics.median(rates),
         "unit": "cell-updates/s",
-        "cores": threads,
+        "cores": cores,
         "kind": "reference",
-        "sample": f"{nsteps} FP64 steps of helium {case.N[0]}x{case.N[1]} from the initial state, "
-                  f"reference dim::advance_one_step, OpenMP guided,1, {secs:.1f} s",
+        "sample": f"{desc}: {steps} step(s) per run from the initial state, reference dim::advance_one_step, "
+                  f"{warmup} warm-up + median of {repeats}, {cores} threads on {cores} physical cores "
+                  f"({cpus} CPUs in the affinity mask), OMP_PROC_BIND=close OMP_PLACES=cores OMP_DYNAMIC=FALSE "
+                  f"OMP_SCHEDULE=guided,1",
+        "runs": rates,
+        "seconds": r["seconds"],
     }
 
 
@@ -128,11 +232,35 @@ def measure(dimhip, workloads, grid, precision, steps, warmup):
     solver.synchronize()
     wall = time.perf_counter() - t0
     ms, counts = solver.kernel_times()
+    dir_ms, dir_counts = solver.sweep_times()
     solver.set_timing(False)
     solver.close()
     if n != steps:
         raise RuntimeError(f"took {n} steps, expected {steps}")
-    return {"case": case, "value": case.ncells * steps / wall, "wall": wall, "ms": ms, "counts": counts}
+    return {"case": case, "value": case.ncells * steps / wall, "wall": wall, "ms": ms, "counts": counts,
+            "dir_ms": dir_ms, "dir_counts": dir_counts}
+
+
+AXES = "xyz"
+
+
+def sweep_roofline(case, precision, ms, dir_ms, traffic=None):
+    """Roofline of the dominant sweep kernel: the slowest of the per-axis
+    sweep launches (they move the same algorithmic bytes), with every axis's
+    figure beside it."""
+    b = sweep_bytes(case, precision)
+    per_axis = {}
+    for d in range(case.dim):
+        a = b / (dir_ms[d] * 1e-3) / 1e9
+        per_axis[AXES[d]] = {"mean_ms": dir_ms[d], "achieved": a, "frac": a / HBM_PEAK_GBS}
+    dom = max(range(case.dim), key=lambda d: dir_ms[d])
+    out = {"bound": "hbm", "achieved": per_axis[AXES[dom]]["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": per_axis[AXES[dom]]["frac"], "traffic": traffic,
+           "kernel": f"{AXES[dom]} sweep (the slowest sweep launch; the last sweep of a step carries the fused CFL "
+                     f"reduction)",
+           "per_axis": per_axis, "mean_ms": {"sweep": ms[0], "boundary": ms[1], "cfl": ms[2]},
+           "bytes_per_launch": b}
+    return out
 
 
 def field_l1(a, b):
@@ -260,19 +388,16 @@ def measure_3d(dimhip, workloads, n, steps, warmup):
         s.synchronize()
         wall = time.perf_counter() - t0
         ms, counts = s.kernel_times()
+        dir_ms, _ = s.sweep_times()
     if nt != steps:
         raise RuntimeError(f"took {nt} steps, expected {steps}")
-    b = sweep_bytes(case, "fp32_mixed")
-    achieved = b / (ms[0] * 1e-3) / 1e9
     return {
         "workload": f"3D bubble collapse {n}^3 (water/air, 1e9 Pa shock), fp32_mixed, split x->y->z, CFL 0.8, "
                     f"{steps} steps, device-generated initial state",
         "value": case.ncells * steps / wall,
         "unit": "cell-updates/s",
         "ms_per_step": wall / steps * 1e3,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "mean_ms": {"sweep": ms[0], "boundary": ms[1], "cfl": ms[2]}, "bytes_per_launch": b},
+        "roofline": sweep_roofline(case, "fp32_mixed", ms, dir_ms),
     }
 
 
@@ -368,12 +493,14 @@ def measure_batch(dimhip, workloads, count, steps):
         b.set_timing(True)
         b.advance(20, 0.8, use_graph=False)
         ms, counts = b.kernel_times()
+        dir_ms, _ = b.sweep_times()
         b.set_timing(False)
     if min(nsteps) != steps or max(nsteps) != steps:
         raise RuntimeError("batched instances did not all take the requested steps")
     per_cell = 2 * (8 * 2 + 4 * 4)  # FP64 masses + FP32 momentum, energy, fraction, read and written
     sweep_bytes_avg = per_cell * cells
-    achieved = sweep_bytes_avg / (ms[0] * 1e-3) / 1e9
+    dom = 0 if dir_ms[0] >= dir_ms[1] else 1
+    achieved = sweep_bytes_avg / (dir_ms[dom] * 1e-3) / 1e9
     return {
         "workload": f"{count} seeded shock-interface instances, 256..1024 x 256..512, FP32-mixed, "
                     f"{steps} steps each, one launch per stage, hipGraph-captured steps",
@@ -384,6 +511,8 @@ def measure_batch(dimhip, workloads, count, steps):
         "autotuned": tune["best"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
+                     "kernel": f"batched {AXES[dom]} sweep (the slower of the two persistent sweep launches)",
+                     "per_axis_ms": {"x": dir_ms[0], "y": dir_ms[1]},
                      "mean_ms": {"sweep": ms[0], "boundary": ms[1], "cfl": ms[2]},
                      "bytes_per_launch": sweep_bytes_avg},
     }
@@ -391,6 +520,9 @@ def measure_batch(dimhip, workloads, count, steps):
 
 def main():
     args = parse()
+    if args.cpu_worker:
+        cpu_worker(json.loads(args.cpu_worker))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "bench.py measures one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
@@ -427,9 +559,15 @@ def main():
     wall = main_run["wall"]
     ms = main_run["ms"]
     counts = main_run["counts"]
-    sweep_ms = ms[0]
-    bytes_per_launch = sweep_bytes(case, precision)
-    achieved = bytes_per_launch / (sweep_ms * 1e-3) / 1e9
+    roof = sweep_roofline(case, precision, ms, main_run["dir_ms"])
+    dom = roof["kernel"][0]
+    roof["traffic"] = traffic_for(args.workload, dom)
+    roof["launches"] = {"sweep": counts[0], "boundary": counts[1], "cfl": counts[2]}
+    roof["timed_steps"] = f"1 in {TIMING_EVERY}"
+    valu = valu_for(args.workload)
+    if valu:
+        roof["limiter"] = valu.get("limiter")
+        roof["valu"] = valu.get(f"{dom}_sweep")
     out = {
         "metric": "cell-updates/s, 2D DIM MUSCL-Hancock+HLLC, FP32-mixed vs FP64; % HBM peak",
         "value": value,
@@ -450,39 +588,37 @@ def main():
             "cells": cells,
             "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_for(args.workload),
-            "kernel": "sweep (x and y launches averaged)",
-            "mean_ms": {"sweep": ms[0], "boundary": ms[1], "cfl": ms[2]},
-            "launches": {"sweep": counts[0], "boundary": counts[1], "cfl": counts[2]},
-            "timed_steps": f"1 in {TIMING_EVERY}",
-            "bytes_per_launch": bytes_per_launch,
-        },
+        "roofline": roof,
         "cpu_baseline": None,
     }
     if args.fp32_steps > 0 and precision == "fp64":
         g32 = WORKLOADS["helium_fp32"][0]
         progress(f"fp32_mixed {g32[0]}x{g32[1]}: {args.fp32_steps} steps")
         r32 = measure(dimhip, workloads, g32, "fp32_mixed", args.fp32_steps, args.warmup)
-        b32 = sweep_bytes(r32["case"], "fp32_mixed")
-        a32 = b32 / (r32["ms"][0] * 1e-3) / 1e9
+        roof32 = sweep_roofline(r32["case"], "fp32_mixed", r32["ms"], r32["dir_ms"])
+        roof32["traffic"] = traffic_for("helium_fp32", roof32["kernel"][0])
+        v32 = valu_for("helium_fp32")
+        if v32:
+            roof32["limiter"] = v32.get("limiter")
+            roof32["valu"] = v32.get(f"{roof32['kernel'][0]}_sweep")
         out["fp32_mixed"] = {
             "workload": f"helium shock-bubble {g32[0]}x{g32[1]}, fp32_mixed, split x->y, CFL 0.8",
             "value": r32["value"],
             "unit": "cell-updates/s",
             "steps": args.fp32_steps,
             "ms_per_step": r32["wall"] / args.fp32_steps * 1e3,
-            "speedup_vs_fp64_cell_rate": r32["value"] / value,
-            "roofline": {"bound": "hbm", "achieved": a32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": a32 / HBM_PEAK_GBS, "traffic": traffic_for("helium_fp32"),
-                         "mean_ms": {"sweep": r32["ms"][0], "boundary": r32["ms"][1], "cfl": r32["ms"][2]},
-                         "bytes_per_launch": b32},
+            "roofline": roof32,
         }
+        if args.fp64_c3_steps > 0:
+            progress(f"fp64 {g32[0]}x{g32[1]}: {args.fp64_c3_steps} steps (the FP32-mixed ratio on one grid)")
+            r64 = measure(dimhip, workloads, g32, "fp64", args.fp64_c3_steps, args.warmup)
+            out["fp32_mixed"]["fp64_same_grid"] = {
+                "workload": f"helium shock-bubble {g32[0]}x{g32[1]}, fp64, split x->y, CFL 0.8",
+                "value": r64["value"], "steps": args.fp64_c3_steps,
+                "ms_per_step": r64["wall"] / args.fp64_c3_steps * 1e3,
+                "roofline": sweep_roofline(r64["case"], "fp64", r64["ms"], r64["dir_ms"]),
+            }
+            out["fp32_mixed"]["speedup_vs_fp64_same_grid"] = r32["value"] / r64["value"]
         acc_steps = args.fp32_steps if args.accuracy_steps < 0 else args.accuracy_steps
         if acc_steps > 0:
             progress(f"fp32_mixed vs fp64 accuracy: {acc_steps} steps of each")
@@ -510,8 +646,14 @@ def main():
         progress("1D water/air tube full run")
         out["water_air_tube_1d"] = measure_tube(dimhip, workloads, not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
-        progress(f"reference CPU solver: {args.cpu_steps} steps of helium 2048x1024")
-        out["cpu_baseline"] = cpu_baseline(workloads.helium_shock_bubble_2d(2048, 1024), args.cpu_steps)
+        progress("reference CPU solver: helium 2048x1024")
+        out["cpu_baseline"] = cpu_baseline("helium_2048x1024", args.cpu_steps)
+        if args.cpu_all and precision == "fp64":
+            others = {}
+            for name in ("helium_4096x2048", "batch_64", "bubble_collapse_128"):
+                progress(f"reference CPU solver: {name}")
+                others[name] = cpu_baseline(name)
+            out["cpu_baseline"]["other_configs"] = others
     print(json.dumps(out))
 
 

@@ -23,7 +23,8 @@ namespace dimhip {
 // sweep tiles come from the policy: Pol::kSweepL0 positions x Pol::kSweepW0
 // lines for x, Pol::kSweepW1 lines (along x) x Pol::kSweepL1 positions for y/z
 
-enum KernelClass { KC_SWEEP = 0, KC_BC = 1, KC_CFL = 2, KC_COUNT = 3 };
+// KC_SWEEP covers every sweep launch; KC_SWEEP_X + d the launches along axis d
+enum KernelClass { KC_SWEEP = 0, KC_BC = 1, KC_CFL = 2, KC_SWEEP_X = 3, KC_COUNT = 6 };
 
 struct Ops;
 
@@ -82,8 +83,8 @@ struct dimhip_ctx {
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, int>> ev_pending;  // (class, index of start event)
     size_t ev_next = 0;
-    double kt_sum[dimhip::KC_COUNT] = {0, 0, 0};
-    long long kt_cnt[dimhip::KC_COUNT] = {0, 0, 0};
+    double kt_sum[dimhip::KC_COUNT] = {};
+    long long kt_cnt[dimhip::KC_COUNT] = {};
 };
 
 namespace dimhip {

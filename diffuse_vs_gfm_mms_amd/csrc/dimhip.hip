@@ -61,11 +61,12 @@ int ev_begin(dimhip_ctx* c)
     return i;
 }
 
-void ev_end(dimhip_ctx* c, int i, int cls)
+void ev_end(dimhip_ctx* c, int i, int cls, int cls2 = -1)
 {
     if (i < 0) return;
     hipEventRecord(c->ev_pool[i + 1], c->stream);
     c->ev_pending.push_back({cls, i});
+    if (cls2 >= 0) c->ev_pending.push_back({cls2, i});
 }
 
 void ev_collect(dimhip_ctx* c)
@@ -310,7 +311,7 @@ void enqueue_step(dimhip_ctx* c, double cfl, double dt_max_host)
             const bool last = dir == D - 1;
             int e = ev_begin(c);
             c->ops->sweep(c, dir, 0, fuse && last, in, out, dir == 0 ? ps : nullptr, fuse && last ? ps : nullptr);
-            ev_end(c, e, KC_SWEEP);
+            ev_end(c, e, KC_SWEEP, KC_SWEEP_X + dir);
             e = ev_begin(c);
             c->ops->bc(c, out, last, fuse && last ? ps : nullptr);
             ev_end(c, e, KC_BC);
@@ -323,7 +324,7 @@ void enqueue_step(dimhip_ctx* c, double cfl, double dt_max_host)
         for (int dir = 0; dir < D; ++dir) {
             const int e = ev_begin(c);
             c->ops->sweep(c, dir, 1, false, base, acc, ps, nullptr);
-            ev_end(c, e, KC_SWEEP);
+            ev_end(c, e, KC_SWEEP, KC_SWEEP_X + dir);
         }
         const int e = ev_begin(c);
         c->ops->bc(c, acc, 1, nullptr);
@@ -651,8 +652,10 @@ int dimhip_advance(dimhip_ctx* c, int nsteps, double cfl, double t_end, int use_
         // Steps go out in chunks; after each chunk the step-control block is
         // copied back asynchronously, and once the device time has reached
         // t_end (or an error was flagged) no further chunk is enqueued.  The
-        // check lags one chunk so the queue never drains: at most one chunk of
-        // (inactive, pass-through) steps is issued past t_end.
+        // check lags one chunk so the queue never drains: up to two chunks of
+        // inactive steps (less than 2 * kChunk) can be issued past t_end.
+        // They are launches that return at once: a 2D split step's inactive
+        // sweeps copy nothing (sweep_tile).
         constexpr int kChunk = 256;
         bool probing = false;
         while (nsteps > 0) {
@@ -921,9 +924,22 @@ int dimhip_kernel_times(dimhip_ctx* c, double* mean_ms, long long* counts)
     if (!c || !mean_ms) return fail(c, "dimhip_kernel_times: null argument");
     HIP_OK(c, hipStreamSynchronize(c->stream));
     ev_collect(c);
-    for (int k = 0; k < KC_COUNT; ++k) {
+    for (int k = 0; k < KC_SWEEP_X; ++k) {
         mean_ms[k] = c->kt_cnt[k] ? c->kt_sum[k] / (double)c->kt_cnt[k] : 0.0;
         if (counts) counts[k] = c->kt_cnt[k];
+    }
+    return 0;
+}
+
+int dimhip_sweep_times(dimhip_ctx* c, double* mean_ms, long long* counts)
+{
+    if (!c || !mean_ms) return fail(c, "dimhip_sweep_times: null argument");
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    ev_collect(c);
+    for (int d = 0; d < 3; ++d) {
+        const int k = KC_SWEEP_X + d;
+        mean_ms[d] = c->kt_cnt[k] ? c->kt_sum[k] / (double)c->kt_cnt[k] : 0.0;
+        if (counts) counts[d] = c->kt_cnt[k];
     }
     return 0;
 }
@@ -1044,8 +1060,8 @@ struct dimhip_batch {
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, int>> ev_pending;
     size_t ev_next = 0;
-    double kt_sum[3] = {0, 0, 0};
-    long long kt_cnt[3] = {0, 0, 0};
+    double kt_sum[KC_COUNT] = {};
+    long long kt_cnt[KC_COUNT] = {};
     std::string err;
 };
 
@@ -1080,11 +1096,12 @@ int bev_begin(dimhip_batch* b)
     return i;
 }
 
-void bev_end(dimhip_batch* b, int i, int cls)
+void bev_end(dimhip_batch* b, int i, int cls, int cls2 = -1)
 {
     if (i < 0) return;
     hipEventRecord(b->ev_pool[i + 1], b->stream);
     b->ev_pending.push_back({cls, i});
+    if (cls2 >= 0) b->ev_pending.push_back({cls2, i});
 }
 
 void bev_collect(dimhip_batch* b)
@@ -1200,14 +1217,14 @@ void batch_enqueue_step(dimhip_batch* b, double cfl)
     const int bcb = std::max(1, std::min(b->nwork[2], 2048));
     int e = bev_begin(b);
     batch_sweep_x(b);
-    bev_end(b, e, KC_SWEEP);
+    bev_end(b, e, KC_SWEEP, KC_SWEEP_X);
     b->cur ^= 1;
     e = bev_begin(b);
     batch_bc_kernel<BPol><<<bcb, 256, 0, b->stream>>>(b->d_insts, b->d_work[2], b->nwork[2], b->cur, 0);
     bev_end(b, e, KC_BC);
     e = bev_begin(b);
     batch_sweep_y(b, b->fusable);
-    bev_end(b, e, KC_SWEEP);
+    bev_end(b, e, KC_SWEEP, KC_SWEEP_X + 1);
     b->cur ^= 1;
     e = bev_begin(b);
     batch_bc_kernel<BPol><<<bcb, 256, 0, b->stream>>>(b->d_insts, b->d_work[2], b->nwork[2], b->cur, 1);
@@ -1366,7 +1383,7 @@ int dimhip_batch_set_timing(dimhip_batch* b, int enable)
     BHIP_OK(b, hipStreamSynchronize(b->stream));
     bev_collect(b);
     b->timing = enable != 0;
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < KC_COUNT; ++k) {
         b->kt_sum[k] = 0.0;
         b->kt_cnt[k] = 0;
     }
@@ -1378,9 +1395,22 @@ int dimhip_batch_kernel_times(dimhip_batch* b, double* mean_ms, long long* count
     if (!b || !mean_ms) return bfail(b, "dimhip_batch_kernel_times: null argument");
     BHIP_OK(b, hipStreamSynchronize(b->stream));
     bev_collect(b);
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < KC_SWEEP_X; ++k) {
         mean_ms[k] = b->kt_cnt[k] ? b->kt_sum[k] / (double)b->kt_cnt[k] : 0.0;
         if (counts) counts[k] = b->kt_cnt[k];
+    }
+    return 0;
+}
+
+int dimhip_batch_sweep_times(dimhip_batch* b, double* mean_ms, long long* counts)
+{
+    if (!b || !mean_ms) return bfail(b, "dimhip_batch_sweep_times: null argument");
+    BHIP_OK(b, hipStreamSynchronize(b->stream));
+    bev_collect(b);
+    for (int d = 0; d < 3; ++d) {
+        const int k = KC_SWEEP_X + d;
+        mean_ms[d] = b->kt_cnt[k] ? b->kt_sum[k] / (double)b->kt_cnt[k] : 0.0;
+        if (counts) counts[d] = b->kt_cnt[k];
     }
     return 0;
 }

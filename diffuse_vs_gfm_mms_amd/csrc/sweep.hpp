@@ -271,9 +271,13 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     const bool own = in && pos >= 3 && pos < L - 3;
 
     if (!ctl->active) {
-        // a step past t_end: the state passes through unchanged, so the
-        // host-side double-buffer flip stays consistent
-        if (MODE == 0 && own) {
+        // a step past t_end (or after an error): the state passes through
+        // unchanged.  The host flips the double buffer once per sweep, D
+        // times per split step; with D even the flips return to the input
+        // buffer, which no inactive sweep writes, so nothing is copied.  With
+        // D odd the first sweep copies and the others, each reading or
+        // writing a buffer that already holds the state, need not.
+        if (MODE == 0 && D % 2 == 1 && DIR == 0 && own) {
             Cell U;
             Pol::template load<D, M>(qin, g.pitch, cell, U);
             Pol::template store<D, M>(qout, g.pitch, cell, U);

@@ -164,6 +164,7 @@ def load_library():
         "dimhip_synchronize": (C.c_int, [vp]),
         "dimhip_set_timing": (C.c_int, [vp, C.c_int]),
         "dimhip_kernel_times": (C.c_int, [vp, dp, llp]),
+        "dimhip_sweep_times": (C.c_int, [vp, dp, llp]),
         "dimhip_kat": (C.c_int, [C.POINTER(Desc), C.c_int, C.c_int, C.c_int, dp, dp, dp, dp]),
         "dimhip_last_error": (C.c_char_p, [vp]),
         "dimhip_batch_create": (C.c_int, [C.POINTER(Desc), C.c_int, C.POINTER(vp)]),
@@ -175,6 +176,7 @@ def load_library():
         "dimhip_batch_advance": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_int, dp, llp]),
         "dimhip_batch_set_timing": (C.c_int, [vp, C.c_int]),
         "dimhip_batch_kernel_times": (C.c_int, [vp, dp, llp]),
+        "dimhip_batch_sweep_times": (C.c_int, [vp, dp, llp]),
         "dimhip_batch_last_error": (C.c_char_p, [vp]),
     }
     for name, (res, args) in sig.items():
@@ -189,10 +191,10 @@ def exported_symbols():
     return ["dimhip_version", "dimhip_create", "dimhip_destroy", "dimhip_upload", "dimhip_download",
             "dimhip_step", "dimhip_advance", "dimhip_compute_dt", "dimhip_sweep", "dimhip_apply_bc",
             "dimhip_conservation", "dimhip_boundary_flux", "dimhip_track_boundary_flux",
-            "dimhip_boundary_flux_integral", "dimhip_upload_primitive", "dimhip_initialise", "dimhip_output_rows", "dimhip_synchronize", "dimhip_set_timing", "dimhip_kernel_times",
+            "dimhip_boundary_flux_integral", "dimhip_upload_primitive", "dimhip_initialise", "dimhip_output_rows", "dimhip_synchronize", "dimhip_set_timing", "dimhip_kernel_times", "dimhip_sweep_times",
             "dimhip_kat", "dimhip_last_error", "dimhip_batch_create", "dimhip_batch_destroy",
             "dimhip_batch_upload", "dimhip_batch_download", "dimhip_batch_configure", "dimhip_batch_autotune",
-            "dimhip_batch_advance", "dimhip_batch_set_timing", "dimhip_batch_kernel_times",
+            "dimhip_batch_advance", "dimhip_batch_set_timing", "dimhip_batch_kernel_times", "dimhip_batch_sweep_times",
             "dimhip_batch_last_error"]
 
 
@@ -363,6 +365,13 @@ class Solver:
         self._check(self.lib.dimhip_kernel_times(self.h, _dp(ms), cnt))
         return ms, list(cnt)
 
+    def sweep_times(self):
+        """Mean device ms per sweep launch along x, y, z and launch counts."""
+        ms = np.zeros(3)
+        cnt = (C.c_longlong * 3)()
+        self._check(self.lib.dimhip_sweep_times(self.h, _dp(ms), cnt))
+        return ms, list(cnt)
+
 
 def kat(which, dim, materials, count, in0, in1=None, d=0, lambda_model="kapila",
         alpha_source_floor=0.0):
@@ -454,4 +463,10 @@ class Batch:
         ms = np.zeros(3)
         cnt = (C.c_longlong * 3)()
         self._check(self.lib.dimhip_batch_kernel_times(self.h, _dp(ms), cnt))
+        return ms, list(cnt)
+
+    def sweep_times(self):
+        ms = np.zeros(3)
+        cnt = (C.c_longlong * 3)()
+        self._check(self.lib.dimhip_batch_sweep_times(self.h, _dp(ms), cnt))
         return ms, list(cnt)

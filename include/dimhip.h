@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define DIMHIP_VERSION 3
+#define DIMHIP_VERSION 4
 #define DIMHIP_MAX_MAT 4
 
 /* EoS kinds on the path (src/euler/eos_params.hpp:6-13; the config names
@@ -227,6 +227,10 @@ int dimhip_synchronize(dimhip_ctx* ctx);
 int dimhip_set_timing(dimhip_ctx* ctx, int every);
 int dimhip_kernel_times(dimhip_ctx* ctx, double* mean_ms, long long* counts);
 
+/* Mean device duration (ms) of the sweep launches along each axis over the
+ * same measured calls: mean_ms[d], counts[d] for d = 0 (x), 1 (y), 2 (z). */
+int dimhip_sweep_times(dimhip_ctx* ctx, double* mean_ms, long long* counts);
+
 /* Known-answer entry point for the device functions on the path.  `which`:
  *   0 cons_to_prim   (primitives.hpp:111-162)  in0: count*nv cells   -> out0: count*(2nmat+dim+1)
  *   1 prim_to_cons   (primitives.hpp:166-212)  in0: count*(2nmat+dim+1) -> out0: count*nv
@@ -278,6 +282,7 @@ int dimhip_batch_advance(dimhip_batch* batch, int nsteps, double cfl, double t_e
 
 int dimhip_batch_set_timing(dimhip_batch* batch, int enable);
 int dimhip_batch_kernel_times(dimhip_batch* batch, double* mean_ms, long long* counts);
+int dimhip_batch_sweep_times(dimhip_batch* batch, double* mean_ms, long long* counts);
 const char* dimhip_batch_last_error(const dimhip_batch* batch);
 
 #ifdef __cplusplus

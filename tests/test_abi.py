@@ -28,4 +28,4 @@ def test_library_exports_declared_symbols():
 
 def test_version_without_gpu():
     lib = dimhip.load_library()
-    assert lib.dimhip_version() == 3
+    assert lib.dimhip_version() == 4

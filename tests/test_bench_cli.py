@@ -44,13 +44,25 @@ def test_3d_sweep_bytes():
 
 @pytest.mark.parametrize("workload, algorithmic", [("helium_fp64", 201326592), ("helium_fp32", 536870912)])
 def test_traffic_summaries(workload, algorithmic):
-    traffic = bench.traffic_for(workload)
-    assert isinstance(traffic, int)
+    """The committed counter summaries give each sweep axis's HBM bytes per
+    launch, at least the algorithmic bytes, and bench.py reads the axis it
+    is asked for."""
     with open(os.path.join(ROOT, "profiles", f"traffic_{workload}.json")) as f:
         t = json.load(f)
     assert t["algorithmic_bytes_per_sweep_launch"] == algorithmic
-    assert t["bytes_per_sweep_launch"] == traffic
-    # the mean of the x and y launches, and at least the algorithmic bytes
-    assert abs(traffic - (t["x_sweep_bytes"] + t["y_sweep_bytes"]) / 2) <= 1
-    assert traffic >= algorithmic
-    assert abs(t["ratio_to_algorithmic"] - traffic / algorithmic) < 1e-3
+    for axis in "xy":
+        traffic = bench.traffic_for(workload, axis)
+        assert isinstance(traffic, int)
+        assert traffic == t[f"{axis}_sweep_bytes"]
+        assert traffic >= algorithmic
+    assert bench.traffic_for(workload, "z") is None
+
+
+def test_dominant_sweep_roofline():
+    """The line's roofline is the slowest sweep launch's, with every axis beside it."""
+    case = W.helium_shock_bubble_2d(64, 32)
+    r = bench.sweep_roofline(case, "fp64", [0.5, 0.01, 0.0], [0.4, 0.6, 0.0])
+    b = bench.sweep_bytes(case, "fp64")
+    assert r["kernel"].startswith("y sweep")
+    assert abs(r["achieved"] - b / 0.6e-3 / 1e9) < 1e-9
+    assert r["frac"] == r["per_axis"]["y"]["frac"] < r["per_axis"]["x"]["frac"]
