@@ -14,7 +14,7 @@ import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # the C ABI and one translation unit per arithmetic policy, compiled in parallel
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("dimhip.hip", "ops_fp64.hip", "ops_fp64na.hip", "ops_fp32.hip")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("dimhip.hip", "ops_fp64.hip", "ops_fp64na.hip", "ops_fp32.hip", "ops_fp32i.hip")]
 OUT_DIR = os.path.join(HERE, "_lib")
 OUT = os.path.join(OUT_DIR, "libdimhip.so")
 

@@ -36,6 +36,8 @@ struct dimhip_ctx {
     dimhip::EosParams eos{};
     dimhip::EosParamsNA eos_na{};  // the same constants, for kernels compiled with the noble-Abel law
     dimhip::EosMixed eosm{};
+    dimhip::EosMixedIdeal eosmi{};  // eosm, for the ideal-gas Kapila kernels
+    bool fp32_ideal = false;        // FP32-mixed with ideal gases only and the Kapila lambda model
     dimhip::BcArgs bc{};
     int nv = 0;
     size_t cell_bytes = 0;  // storage bytes per cell

@@ -56,6 +56,21 @@ struct Grid {
     double dx[3];
 };
 
+// Element `off` (a 32-bit count) of a plane whose pointer is uniform across
+// the wave: the byte offset is formed in 32 bits, so the access can take the
+// scalar base + 32-bit vector offset addressing mode (no 64-bit address per
+// lane).
+template<typename T>
+__device__ __forceinline__ const T& plane_at(const T* plane, unsigned off)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(plane) + off * (unsigned)sizeof(T));
+}
+template<typename T>
+__device__ __forceinline__ T& plane_at(T* plane, unsigned off)
+{
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(plane) + off * (unsigned)sizeof(T));
+}
+
 // Per-context step control block, resident in device memory.  `dtmin` holds
 // the bit patterns of positive doubles (ordered like unsigned integers) for
 // an atomic min; slot `slot` is the current step's CFL minimum, the other

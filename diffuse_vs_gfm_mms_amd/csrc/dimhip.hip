@@ -90,7 +90,9 @@ struct OpsImpl final : Ops {
     void sweep(dimhip_ctx* c, int dir, int mode, bool fuse, const void* in, void* out, const double* pin,
                double* pout) const override
     {
-        if (c->desc.precision != DIMHIP_FP64)
+        if (c->desc.precision != DIMHIP_FP64 && c->fp32_ideal)
+            PolOps<Fp32Ideal, D, M>::sweep(c, dir, mode, fuse, in, out, nullptr, nullptr, c->eosmi);
+        else if (c->desc.precision != DIMHIP_FP64)
             PolOps<Fp32Mixed, D, M>::sweep(c, dir, mode, fuse, in, out, nullptr, nullptr, c->eosm);
         else if (c->eos.noble_abel) PolOps<Fp64NA, D, M>::sweep(c, dir, mode, fuse, in, out, pin, pout, c->eos_na);
         else PolOps<Fp64, D, M>::sweep(c, dir, mode, fuse, in, out, pin, pout, c->eos);
@@ -98,14 +100,18 @@ struct OpsImpl final : Ops {
 
     void bc(dimhip_ctx* c, void* q, int end_step, double* pstore) const override
     {
-        if (c->desc.precision != DIMHIP_FP64) PolOps<Fp32Mixed, D, M>::bc(c, q, end_step, nullptr, c->eosm);
+        if (c->desc.precision != DIMHIP_FP64 && c->fp32_ideal)
+            PolOps<Fp32Ideal, D, M>::bc(c, q, end_step, nullptr, c->eosmi);
+        else if (c->desc.precision != DIMHIP_FP64) PolOps<Fp32Mixed, D, M>::bc(c, q, end_step, nullptr, c->eosm);
         else if (c->eos.noble_abel) PolOps<Fp64NA, D, M>::bc(c, q, end_step, pstore, c->eos_na);
         else PolOps<Fp64, D, M>::bc(c, q, end_step, pstore, c->eos);
     }
 
     void cfl(dimhip_ctx* c, const void* q, bool unsplit, double cfl, double* pout) const override
     {
-        if (c->desc.precision != DIMHIP_FP64) PolOps<Fp32Mixed, D, M>::cfl(c, q, unsplit, cfl, nullptr, c->eosm);
+        if (c->desc.precision != DIMHIP_FP64 && c->fp32_ideal)
+            PolOps<Fp32Ideal, D, M>::cfl(c, q, unsplit, cfl, nullptr, c->eosmi);
+        else if (c->desc.precision != DIMHIP_FP64) PolOps<Fp32Mixed, D, M>::cfl(c, q, unsplit, cfl, nullptr, c->eosm);
         else if (c->eos.noble_abel) PolOps<Fp64NA, D, M>::cfl(c, q, unsplit, cfl, pout, c->eos_na);
         else PolOps<Fp64, D, M>::cfl(c, q, unsplit, cfl, pout, c->eos);
     }
@@ -215,6 +221,10 @@ int validate_desc(const dimhip_desc* d, std::string& msg)
                 return msg = "dimhip: nonreflective boundaries need at least 4 cells", 1;
         }
     }
+    // the x and y sweeps address a tile's cells with 32-bit byte offsets:
+    // up to 128 rows of N[0] doubles
+    if (d->dim >= 2 && (long long)d->N[0] * 128 * 8 >= (1ll << 32))
+        return msg = "dimhip: grid extent N[0] above 4194303 is not supported", 1;
     if (d->lambda_model < 0 || d->lambda_model > 2) return msg = "dim::alpha_rhs_coefficients: unknown lambda model", 1;
     if (d->time_update != DIMHIP_SPLIT && d->time_update != DIMHIP_UNSPLIT) return msg = "dimhip: unknown time_update", 1;
     if (d->precision != DIMHIP_FP64 && d->precision != DIMHIP_FP32_MIXED) return msg = "dimhip: unknown precision", 1;
@@ -459,6 +469,8 @@ int dimhip_create(const dimhip_desc* desc, dimhip_ctx** out)
         if (c->eosm.b_ghost[k] != 0.0f) c->eosm.stiff_ghost = 1;
     }
     c->eosm.alpha_source_floor = (float)desc->alpha_source_floor;
+    c->eosmi = EosMixedIdeal(c->eosm);
+    c->fp32_ideal = !c->eosm.stiff && desc->lambda_model == DIMHIP_LAMBDA_KAPILA;
     // ring cells per face (d, side), owned by the face of their lowest boundary axis
     long long acc = 0;
     for (int f = 0; f < 6; ++f) {
@@ -1029,8 +1041,12 @@ int dimhip_kat(const dimhip_desc* desc, int which, int dir, int count, const dou
 // ===========================================================================
 namespace {
 
+// The instance table is built once as BatchInst<Fp32Mixed>; when every
+// instance is ideal-gas Kapila the launches take the Fp32Ideal kernels over
+// the same table (its parameter block is layout-identical).
 using BPol = Fp32Mixed;
 using BInst = BatchInst<BPol>;
+static_assert(sizeof(BatchInst<Fp32Ideal>) == sizeof(BInst), "layout-identical instance tables");
 
 // tile-shape variants of the batched sweeps (autotuned)
 constexpr int kBatchXL[2] = {128, 256};  // x sweep: positions per line; 256/L lines per block
@@ -1047,6 +1063,7 @@ struct dimhip_batch {
     Work* d_work[4] = {nullptr, nullptr, nullptr, nullptr};  // x, y, bc, cfl
     int nwork[4] = {0, 0, 0, 0};
     int xv = 0, yv = 0, schedule = 1, blocks_per_cu = 4;
+    bool ideal = false;  // every instance ideal-gas Kapila: the Fp32Ideal kernels
     int n_cu = 256;
     int cur = 0;
     bool dt_ready = false;
@@ -1177,12 +1194,37 @@ int sweep_blocks(const dimhip_batch* b, int nwork)
     return std::max(1, std::min(nwork, b->n_cu * b->blocks_per_cu));
 }
 
+template<typename Pol>
+const BatchInst<Pol>* insts_as(const dimhip_batch* b)
+{
+    return reinterpret_cast<const BatchInst<Pol>*>(b->d_insts);
+}
+
 template<int DIR, int L, int W, bool FUSE>
 void launch_bsweep(dimhip_batch* b, int k)
 {
     const dim3 blk = DIR == 0 ? dim3(L, W, 1) : dim3(W, L, 1);
-    batch_sweep_kernel<BPol, DIR, L, W, FUSE><<<sweep_blocks(b, b->nwork[k]), blk, 0, b->stream>>>(
-        b->d_insts, b->d_work[k], b->nwork[k], b->schedule, b->cur);
+    const int g = sweep_blocks(b, b->nwork[k]);
+    if (b->ideal)
+        batch_sweep_kernel<Fp32Ideal, DIR, L, W, FUSE><<<g, blk, 0, b->stream>>>(
+            insts_as<Fp32Ideal>(b), b->d_work[k], b->nwork[k], b->schedule, b->cur);
+    else
+        batch_sweep_kernel<BPol, DIR, L, W, FUSE><<<g, blk, 0, b->stream>>>(
+            b->d_insts, b->d_work[k], b->nwork[k], b->schedule, b->cur);
+}
+
+template<typename Pol>
+void launch_bcfl(dimhip_batch* b)
+{
+    batch_cfl_kernel<Pol><<<std::max(1, std::min(b->nwork[3], 4096)), 256, 0, b->stream>>>(
+        insts_as<Pol>(b), b->d_work[3], b->nwork[3], b->cur);
+}
+
+template<typename Pol>
+void launch_bbc(dimhip_batch* b, int end_step)
+{
+    const int bcb = std::max(1, std::min(b->nwork[2], 2048));
+    batch_bc_kernel<Pol><<<bcb, 256, 0, b->stream>>>(insts_as<Pol>(b), b->d_work[2], b->nwork[2], b->cur, end_step);
 }
 
 void batch_sweep_x(dimhip_batch* b)
@@ -1209,25 +1251,26 @@ void batch_enqueue_step(dimhip_batch* b, double cfl)
     if (!b->dt_ready || b->dt_ready_cfl != cfl) {
         const int e = bev_begin(b);
         batch_begin_step_kernel<BPol><<<n, kDtSlots, 0, b->stream>>>(b->d_insts, n, 1);
-        batch_cfl_kernel<BPol><<<std::max(1, std::min(b->nwork[3], 4096)), 256, 0, b->stream>>>(
-            b->d_insts, b->d_work[3], b->nwork[3], b->cur);
+        if (b->ideal) launch_bcfl<Fp32Ideal>(b);
+        else launch_bcfl<BPol>(b);
         bev_end(b, e, KC_CFL);
     }
     batch_begin_step_kernel<BPol><<<n, kDtSlots, 0, b->stream>>>(b->d_insts, n, 0);
-    const int bcb = std::max(1, std::min(b->nwork[2], 2048));
     int e = bev_begin(b);
     batch_sweep_x(b);
     bev_end(b, e, KC_SWEEP, KC_SWEEP_X);
     b->cur ^= 1;
     e = bev_begin(b);
-    batch_bc_kernel<BPol><<<bcb, 256, 0, b->stream>>>(b->d_insts, b->d_work[2], b->nwork[2], b->cur, 0);
+    if (b->ideal) launch_bbc<Fp32Ideal>(b, 0);
+    else launch_bbc<BPol>(b, 0);
     bev_end(b, e, KC_BC);
     e = bev_begin(b);
     batch_sweep_y(b, b->fusable);
     bev_end(b, e, KC_SWEEP, KC_SWEEP_X + 1);
     b->cur ^= 1;
     e = bev_begin(b);
-    batch_bc_kernel<BPol><<<bcb, 256, 0, b->stream>>>(b->d_insts, b->d_work[2], b->nwork[2], b->cur, 1);
+    if (b->ideal) launch_bbc<Fp32Ideal>(b, 1);
+    else launch_bbc<BPol>(b, 1);
     bev_end(b, e, KC_BC);
     b->dt_ready = b->fusable;
     b->dt_ready_cfl = cfl;
@@ -1257,6 +1300,8 @@ int dimhip_batch_create(const dimhip_desc* descs, int count, dimhip_batch** out)
             return bfail(nullptr, std::string("instance ") + std::to_string(i) + ": " + g_global_err);
         b->ctx.push_back(c);
     }
+    b->ideal = true;
+    for (auto* c : b->ctx) b->ideal = b->ideal && c->fp32_ideal;
     BHIP_OK(nullptr, hipSetDevice(descs[0].device));
     hipDeviceProp_t prop;
     BHIP_OK(nullptr, hipGetDeviceProperties(&prop, descs[0].device));

@@ -43,6 +43,23 @@ struct EosMixed {
     int stiff_ghost;          // any b_ghost != 0
 };
 
+// The same parameter block for contexts whose materials are all ideal gases
+// under the Kapila lambda model (the helium shock-bubble, the batched
+// shock-interface instances): the kernels instantiated with it drop the
+// stiffening and lambda-model branches at compile time.
+struct EosMixedIdeal : EosMixed {
+    EosMixedIdeal() = default;
+    explicit EosMixedIdeal(const EosMixed& e) : EosMixed(e) {}
+};
+static_assert(sizeof(EosMixedIdeal) == sizeof(EosMixed), "layout-identical parameter blocks");
+
+__device__ __forceinline__ bool law_stiff(const EosMixed& P) { return P.stiff != 0; }
+__device__ __forceinline__ constexpr bool law_stiff(const EosMixedIdeal&) { return false; }
+__device__ __forceinline__ bool law_stiff_ghost(const EosMixed& P) { return P.stiff_ghost != 0; }
+__device__ __forceinline__ constexpr bool law_stiff_ghost(const EosMixedIdeal&) { return false; }
+__device__ __forceinline__ int law_lambda(const EosMixed& P) { return P.lambda_model; }
+__device__ __forceinline__ constexpr int law_lambda(const EosMixedIdeal&) { return LAMBDA_KAPILA; }
+
 namespace f32 {
 
 __device__ __forceinline__ float fmaxs(float a, float b) { return __builtin_fmaxf(a, b); }
@@ -74,7 +91,8 @@ __device__ __forceinline__ float safe_div_f(float n, float d, float tol = 1e-10f
 //   w_k/(gamma_k - 1) = min(1/(gamma_k - 1), max(1e10 rho_k, 1))
 //                     = med3(1e10 rho_k, min(1, 1/(gamma_k - 1)), 1/(gamma_k - 1)).
 // Materials with gamma_k >= 2 always have w_k = 1.
-__device__ __forceinline__ float ecoef(const EosMixed& P, int k, float rho_k)
+template<typename EP>
+__device__ __forceinline__ float ecoef(const EP& P, int k, float rho_k)
 {
     return __builtin_amdgcn_fmed3f(rho_k * 1e10f, P.coef_lo[k], P.inv_gm1[k]);
 }
@@ -83,17 +101,18 @@ __device__ __forceinline__ float ecoef(const EosMixed& P, int k, float rho_k)
 // the pressure recomputed from the energy built at the clamped density,
 // p' = clamp_min(w_k (p + gamma p_inf) - gamma p_inf), and
 // rho' c^2 = gamma (p' + p_inf); coef = ecoef(rho_k)
-__device__ __forceinline__ float impedance(const EosMixed& P, int k, float p, float coef)
+template<typename EP>
+__device__ __forceinline__ float impedance(const EP& P, int k, float p, float coef)
 {
     const float w = coef * P.gm1[k];
-    if (P.stiff) return P.gamma[k] * (fmaxs(w * (p + P.gpinf[k]) - P.gpinf[k], 1e-10f) + P.pinf[k]);
+    if (law_stiff(P)) return P.gamma[k] * (fmaxs(w * (p + P.gpinf[k]) - P.gpinf[k], 1e-10f) + P.pinf[k]);
     return P.gamma[k] * fmaxs(w * p, 1e-10f);
 }
 
 // sum_k alpha_k (1 - w_k) b_k: the stiffening energy a down-weighted fraction
 // does not carry (only for stiffened materials with gamma < 2)
-template<int M>
-__device__ __forceinline__ float ghost_shift(const EosMixed& P, const float* alpha, const float* coef)
+template<int M, typename EP>
+__device__ __forceinline__ float ghost_shift(const EP& P, const float* alpha, const float* coef)
 {
     float s = -0.f;
 #pragma unroll
@@ -204,20 +223,20 @@ __device__ __forceinline__ void encode_alpha(const float* alpha, float* a)
 // with alpha the decoded stored volume fractions, so that
 // rho e - B = sum_k alpha_k p/(gamma_k - 1) carries no cancellation between
 // terms of size gamma p_inf and the pressure (water: 7.8e8 J/m^3 vs 1e5 Pa).
-template<int M>
-__device__ __forceinline__ float bshift(const EosMixed& P, const float* from, const float* to)
+template<int M, typename EP>
+__device__ __forceinline__ float bshift(const EP& P, const float* from, const float* to)
 {
-    if (!P.stiff) return -0.f;
+    if (!law_stiff(P)) return -0.f;
     float s = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) s += P.b[k] * (from[k] - to[k]);
     return s;
 }
 
-template<int M>
-__device__ __forceinline__ float bdot(const EosMixed& P, const float* alpha)
+template<int M, typename EP>
+__device__ __forceinline__ float bdot(const EP& P, const float* alpha)
 {
-    if (!P.stiff) return -0.f;
+    if (!law_stiff(P)) return -0.f;
     float s = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) s += P.b[k] * alpha[k];
@@ -260,33 +279,33 @@ __device__ __forceinline__ bool interface_cell(const float* alpha)
 // p from the shifted internal energy density
 // (rho e - B(alpha)) = sum_k alpha_k w_k p/(gamma_k-1) - sum_k alpha_k (1 - w_k) b_k,
 // coef[k] = ecoef of the material's partial density
-template<int M>
-__device__ __forceinline__ float pressure_shifted(const EosMixed& P, float rho_e_hat, const float* alpha,
+template<int M, typename EP>
+__device__ __forceinline__ float pressure_shifted(const EP& P, float rho_e_hat, const float* alpha,
                                                   const float* coef)
 {
     float S = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) S += fmaxs(alpha[k], 0.f) * coef[k];
-    if (P.stiff_ghost) rho_e_hat += ghost_shift<M>(P, alpha, coef);
+    if (law_stiff_ghost(P)) rho_e_hat += ghost_shift<M>(P, alpha, coef);
     return rho_e_hat * rcp(S);
 }
 
 // rho e - B at pressure p: sum_k alpha_k w_k max(p,1e-10)/(gamma_k-1) - sum_k alpha_k (1 - w_k) b_k
-template<int M>
-__device__ __forceinline__ float energy_shifted(const EosMixed& P, float p, const float* alpha, const float* coef)
+template<int M, typename EP>
+__device__ __forceinline__ float energy_shifted(const EP& P, float p, const float* alpha, const float* coef)
 {
     const float pk = fmaxs(p, 1e-10f);
     float s = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) s += fmaxs(alpha[k], 0.f) * (pk * coef[k]);
-    if (P.stiff_ghost) s -= ghost_shift<M>(P, alpha, coef);
+    if (law_stiff_ghost(P)) s -= ghost_shift<M>(P, alpha, coef);
     return s;
 }
 
 // cons_to_prim (primitives.hpp:111-162) with the closed-form pressure, on a
 // state with a valid fraction code (stored, half-step and face states)
-template<int D, int M>
-__device__ __forceinline__ void cons_to_prim(const EosMixed& P, const St<D, M>& U, Pr<D, M>& R)
+template<int D, int M, typename EP>
+__device__ __forceinline__ void cons_to_prim(const EP& P, const St<D, M>& U, Pr<D, M>& R)
 {
     float ad[M];
     decode_alpha<M>(U.a, ad);
@@ -297,7 +316,22 @@ __device__ __forceinline__ void cons_to_prim(const EosMixed& P, const St<D, M>& 
 #pragma unroll
     for (int k = 0; k < M; ++k) rho += U.m[k];
     rho = fmaxs(rho, 1e-10f);
-    const float inv_rho = rcp(rho);
+    float inv_rho, ia[M];
+    if constexpr (M == 2) {
+        // one reciprocal for 1/rho, 1/alpha_0' and 1/alpha_1' (alpha' =
+        // max(alpha, 1e-10), so the product stays above 1e-30)
+        const float a0 = fmaxs(R.alpha[0], 1e-10f), a1 = fmaxs(R.alpha[1], 1e-10f);
+        const float a01 = a0 * a1;
+        const float X = rcp(a01 * rho);
+        const float rX = rho * X;
+        inv_rho = a01 * X;
+        ia[0] = a1 * rX;
+        ia[1] = a0 * rX;
+    } else {
+        inv_rho = rcp(rho);
+#pragma unroll
+        for (int k = 0; k < M; ++k) ia[k] = rcp(fmaxs(R.alpha[k], 1e-10f));
+    }
     float v2 = -0.f;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -307,11 +341,11 @@ __device__ __forceinline__ void cons_to_prim(const EosMixed& P, const St<D, M>& 
     float coef[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-        R.rho[k] = fmaxs(U.m[k] * rcp(fmaxs(R.alpha[k], 1e-10f)), 1e-12f);
+        R.rho[k] = fmaxs(U.m[k] * ia[k], 1e-12f);
         coef[k] = ecoef(P, k, R.rho[k]);
     }
     const float kinetic = 0.5f * rho * v2;
-    const float shift = P.stiff ? bshift<M>(P, ad, R.alpha) : 0.f;
+    const float shift = law_stiff(P) ? bshift<M>(P, ad, R.alpha) : 0.f;
     float p;
     if (interface_cell<M>(R.alpha)) {
         // FP64 recovery: the energy minus kinetic energy difference, where the
@@ -334,8 +368,8 @@ __device__ __forceinline__ void cons_to_prim(const EosMixed& P, const St<D, M>& 
 // prim_to_cons (primitives.hpp:166-212) of a primitive whose fractions are
 // non-negative and sum to one (reconstructed faces after enforce); alpha
 // receives the floored fractions the state was built from
-template<int D, int M>
-__device__ __forceinline__ void prim_to_cons(const EosMixed& P, const Pr<D, M>& Q, St<D, M>& U, float* alpha)
+template<int D, int M, typename EP>
+__device__ __forceinline__ void prim_to_cons(const EP& P, const Pr<D, M>& Q, St<D, M>& U, float* alpha)
 {
 #pragma unroll
     for (int k = 0; k < M; ++k) alpha[k] = Q.alpha[k];
@@ -357,7 +391,7 @@ __device__ __forceinline__ void prim_to_cons(const EosMixed& P, const Pr<D, M>& 
     }
     encode_alpha<M>(alpha, U.a);
     float shift = -0.f;
-    if (P.stiff) {
+    if (law_stiff(P)) {
         float ad[M];
         decode_alpha<M>(U.a, ad);
         shift = bshift<M>(P, alpha, ad);
@@ -370,8 +404,8 @@ __device__ __forceinline__ void prim_to_cons(const EosMixed& P, const Pr<D, M>& 
 // common case (valid fraction code, no inactive mass) is the energy floor
 // alone; fraction renormalisation and the removal of inactive mass with its
 // pressure-preserving rebuild run only when needed.
-template<int D, int M, typename S>
-__device__ __forceinline__ void repair(const EosMixed& P, S& U)
+template<int D, int M, typename S, typename EP>
+__device__ __forceinline__ void repair(const EP& P, S& U)
 {
     using MT = decltype(U.m[0] + 0.f);
     MT m_in[M];
@@ -393,7 +427,7 @@ __device__ __forceinline__ void repair(const EosMixed& P, S& U)
         U.E = ffinite_or(U.E);
         sanitise<M>(alpha, 0.f);
         encode_alpha<M>(alpha, U.a);
-        if (P.stiff) {
+        if (law_stiff(P)) {
             float ad1[M];
             decode_alpha<M>(U.a, ad1);
             e_hat_before = U.E + bshift<M>(P, ad0, alpha);
@@ -511,8 +545,8 @@ __device__ __forceinline__ void reconstruct(const Pr<D, M>& L, const Pr<D, M>& C
 // sum_k alpha_k / (rho_k c_k^2), the Wood compressibility (eos.hpp:195-221),
 // with rho_k c_k^2 = gamma_k (p' + p_inf_k) at the clamped partial density
 // (impedance; gamma_k (p + p_inf_k) unless the fraction is near-massless)
-template<int M>
-__device__ __forceinline__ float compressibility(const EosMixed& P, float p, const float* alpha, const float* rho_k)
+template<int M, typename EP>
+__device__ __forceinline__ float compressibility(const EP& P, float p, const float* alpha, const float* rho_k)
 {
     float beta = -0.f;
 #pragma unroll
@@ -525,12 +559,12 @@ __device__ __forceinline__ float compressibility(const EosMixed& P, float p, con
 
 // Wood sound speed (eos.hpp:195-221) and the allaire / equal_velocity
 // variants (eos.hpp:223-275), closed form for ideal and stiffened gases
-template<int M>
-__device__ __forceinline__ float sound_speed(const EosMixed& P, float rho, float p, const float* alpha, const float* rho_k)
+template<int M, typename EP>
+__device__ __forceinline__ float sound_speed(const EP& P, float rho, float p, const float* alpha, const float* rho_k)
 {
     rho = fmaxs(rho, 1e-10f);
     p = fmaxs(p, 1e-10f);
-    if (P.lambda_model == LAMBDA_KAPILA) {
+    if (law_lambda(P) == LAMBDA_KAPILA) {
         if constexpr (M == 2) {
             // c^2 = 1/(rho (a0/I0 + a1/I1)) = Y/(rho X), X = a0 I1 + a1 I0, Y = I0 I1,
             // so c = Y rsq(rho X Y): one transcendental
@@ -560,7 +594,7 @@ __device__ __forceinline__ float sound_speed(const EosMixed& P, float rho, float
 #pragma unroll
     for (int k = 0; k < M; ++k) {
         const float a = fmaxs(alpha[k], 0.f);
-        const float lam = (P.lambda_model == LAMBDA_EQUAL_VELOCITY) ? rcp(fmaxs(cs[k], 1e-14f) * inv_sum) : 1.f;
+        const float lam = (law_lambda(P) == LAMBDA_EQUAL_VELOCITY) ? rcp(fmaxs(cs[k], 1e-14f) * inv_sum) : 1.f;
         rc2 += lam * a * xi[k] * cs[k] * cs[k];
     }
     return fsqrt(fmaxs(rc2 * rcp(xi_mix) * rcp(rho), 1e-14f));
@@ -569,7 +603,8 @@ __device__ __forceinline__ float sound_speed(const EosMixed& P, float rho, float
 // acoustic_impedance of alpha_rhs_coefficients (primitives.hpp:288-297):
 // max(rho_k c_k^2, 1e-12) with rho_k the primitive's partial density (floored
 // at 1e-12) and c_k from the clamped one (1e-10), rho_k/rho' = min(1, 1e10 rho_k)
-__device__ __forceinline__ float kapila_impedance(const EosMixed& P, int k, float p, float rho_k)
+template<typename EP>
+__device__ __forceinline__ float kapila_impedance(const EP& P, int k, float p, float rho_k)
 {
     return fmaxs(fmins(rho_k * 1e10f, 1.f) * impedance(P, k, p, ecoef(P, k, rho_k)), 1e-12f);
 }
@@ -577,15 +612,15 @@ __device__ __forceinline__ float kapila_impedance(const EosMixed& P, int k, floa
 // alpha_rhs_coefficients (primitives.hpp:216-322)
 // Q comes from cons_to_prim: its fractions are sanitised already, so the
 // reference's second sanitise (floor 0) is the identity here.
-template<int D, int M>
-__device__ __forceinline__ void alpha_rhs(const EosMixed& P, const Pr<D, M>& Q, float* rhs)
+template<int D, int M, typename EP>
+__device__ __forceinline__ void alpha_rhs(const EP& P, const Pr<D, M>& Q, float* rhs)
 {
 #pragma unroll
     for (int k = 0; k < M; ++k) rhs[k] = Q.alpha[k];
-    if (P.lambda_model == LAMBDA_ALLAIRE) return;
+    if (law_lambda(P) == LAMBDA_ALLAIRE) return;
     const float p = fmaxs(Q.p, 1e-12f);
     const float srcf = fmaxs(P.alpha_source_floor, 0.f);
-    if (P.lambda_model == LAMBDA_EQUAL_VELOCITY) {
+    if (law_lambda(P) == LAMBDA_EQUAL_VELOCITY) {
         float cs[M], inv_sum = -0.f;
         bool act[M];
 #pragma unroll
@@ -636,8 +671,8 @@ __device__ __forceinline__ void alpha_rhs(const EosMixed& P, const Pr<D, M>& Q, 
 }
 
 // physical flux (flux.hpp:11-43) with the true energy E = E_hat + B(alpha)
-template<int D, int M, int DIR>
-__device__ __forceinline__ void flux_normal(const EosMixed& P, const St<D, M>& U, const Pr<D, M>& Q, Fx<D, M>& F)
+template<int D, int M, int DIR, typename EP>
+__device__ __forceinline__ void flux_normal(const EP& P, const St<D, M>& U, const Pr<D, M>& Q, Fx<D, M>& F)
 {
     float rho = -0.f;
 #pragma unroll
@@ -648,7 +683,7 @@ __device__ __forceinline__ void flux_normal(const EosMixed& P, const St<D, M>& U
 #pragma unroll
     for (int d = 0; d < D; ++d) F.mom[d] = rho * Q.vel[d] * un + (d == DIR ? Q.p : 0.f);
     float B = -0.f;
-    if (P.stiff) {
+    if (law_stiff(P)) {
         float ad[M];
         decode_alpha<M>(U.a, ad);
         B = bdot<M>(P, ad);
@@ -657,8 +692,8 @@ __device__ __forceinline__ void flux_normal(const EosMixed& P, const St<D, M>& U
 }
 
 // muscl.hpp:143-191
-template<int D, int M, int DIR>
-__device__ __forceinline__ void hancock(const EosMixed& P, const St<D, M>& Uc, const Pr<D, M>& Pc, const St<D, M>& ul,
+template<int D, int M, int DIR, typename EP>
+__device__ __forceinline__ void hancock(const EP& P, const St<D, M>& Uc, const Pr<D, M>& Pc, const St<D, M>& ul,
                                         const St<D, M>& ur, const Pr<D, M>& lf, const Pr<D, M>& rf, const float* rhs,
                                         float dt, float dx, St<D, M>& out)
 {
@@ -680,7 +715,7 @@ __device__ __forceinline__ void hancock(const EosMixed& P, const St<D, M>& Uc, c
     for (int k = 0; k < M; ++k)
         an[k] = Pc.alpha[k] - factor * (rf.alpha[k] * ur_ - lf.alpha[k] * ul_) + 0.5f * dt * rhs[k] * div_u;
     sanitise<M>(an, 0.f);
-    if (P.stiff) {
+    if (law_stiff(P)) {
         float ad_old[M], ad_new[M];
         decode_alpha<M>(out.a, ad_old);
         encode_alpha<M>(an, out.a);
@@ -693,8 +728,8 @@ __device__ __forceinline__ void hancock(const EosMixed& P, const St<D, M>& Uc, c
 }
 
 // hllc.hpp:62-195
-template<int D, int M, int DIR>
-__device__ __forceinline__ int hllc(const EosMixed& P, const St<D, M>& UL, const St<D, M>& UR, const Pr<D, M>& PL,
+template<int D, int M, int DIR, typename EP>
+__device__ __forceinline__ int hllc(const EP& P, const St<D, M>& UL, const St<D, M>& UR, const Pr<D, M>& PL,
                                      const Pr<D, M>& PR, Fx<D, M>& F, float& fv)
 {
     float rhoL = -0.f, rhoR = -0.f;
@@ -756,7 +791,7 @@ __device__ __forceinline__ int hllc(const EosMixed& P, const St<D, M>& UL, const
     // delta = rho*/rho_K - 1 = (s* - u_K)/(s_K - s*) so that E* - E_K carries
     // no cancellation against the stiffened-gas energy
     float BK = -0.f;
-    if (P.stiff) {
+    if (law_stiff(P)) {
         float adK[M];
         decode_alpha<M>(UK.a, adK);
         BK = bdot<M>(P, adK);
@@ -771,7 +806,7 @@ __device__ __forceinline__ int hllc(const EosMixed& P, const St<D, M>& UL, const
     fv = s_star;
     if (rho_star <= 0.f || !ffinite(rho_star) || !ffinite(E_star)) {
         float dB = -0.f;
-        if (P.stiff) {
+        if (law_stiff(P)) {
             float adL[M], adR[M];
             decode_alpha<M>(UL.a, adL);
             decode_alpha<M>(UR.a, adR);
@@ -800,7 +835,8 @@ __device__ __forceinline__ int hllc(const EosMixed& P, const St<D, M>& UL, const
 
 }  // namespace f32
 
-struct Fp32Mixed {
+template<typename EP>
+struct Fp32MixedT {
     // register cap of the sweep kernels: 8 waves per SIMD (64 VGPRs) fits the
     // 1D/2D two-material sweeps without spills; the 3D and three-material
     // sweeps need more registers and run uncapped (a cap spills them)
@@ -820,7 +856,7 @@ struct Fp32Mixed {
     // the recovered pressure is not carried between sweeps (closed form)
     static constexpr bool kStoredPressure = false;
     using R = float;
-    using Eos = EosMixed;
+    using Eos = EP;
     template<int D, int M> using St = f32::St<D, M>;
     template<int D, int M> using Pr = f32::Pr<D, M>;
     template<int D, int M> using Fx = f32::Fx<D, M>;
@@ -841,6 +877,42 @@ struct Fp32Mixed {
         U.E = qf[(f++) * pitch + c];
 #pragma unroll
         for (int k = 0; k < M - 1; ++k) U.a[k] = qf[(f++) * pitch + c];
+    }
+
+    // load / store of cell cb + cr with cb uniform across the wave and cr a
+    // 32-bit offset (cr * 8 < 2^32)
+    template<int D, int M>
+    static __device__ __forceinline__ void load_rel(const void* base, long long pitch, long long cb, unsigned cr,
+                                                    Cell<D, M>& U)
+    {
+        const double* qm = (const double*)base;
+        const float* qf = (const float*)(qm + M * pitch) + cb;
+        qm += cb;
+#pragma unroll
+        for (int k = 0; k < M; ++k) U.m[k] = plane_at(qm + k * pitch, cr);
+        int f = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) U.mom[d] = plane_at(qf + (f++) * pitch, cr);
+        U.E = plane_at(qf + (f++) * pitch, cr);
+#pragma unroll
+        for (int k = 0; k < M - 1; ++k) U.a[k] = plane_at(qf + (f++) * pitch, cr);
+    }
+
+    template<int D, int M>
+    static __device__ __forceinline__ void store_rel(void* base, long long pitch, long long cb, unsigned cr,
+                                                     const Cell<D, M>& U)
+    {
+        double* qm = (double*)base;
+        float* qf = (float*)(qm + M * pitch) + cb;
+        qm += cb;
+#pragma unroll
+        for (int k = 0; k < M; ++k) plane_at(qm + k * pitch, cr) = U.m[k];
+        int f = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) plane_at(qf + (f++) * pitch, cr) = U.mom[d];
+        plane_at(qf + (f++) * pitch, cr) = U.E;
+#pragma unroll
+        for (int k = 0; k < M - 1; ++k) plane_at(qf + (f++) * pitch, cr) = U.a[k];
     }
 
     template<int D, int M>
@@ -1145,5 +1217,8 @@ struct Fp32Mixed {
     template<int D, int M>
     static __device__ __forceinline__ double energy(const Eos& P, const Cell<D, M>& U) { return (double)U.E + Bd<M>(P, U.a); }
 };
+
+using Fp32Mixed = Fp32MixedT<EosMixed>;
+using Fp32Ideal = Fp32MixedT<EosMixedIdeal>;
 
 }  // namespace dimhip

@@ -70,6 +70,38 @@ struct Fp64T {
         for (int k = 0; k < M - 1; ++k) U.a[k] = q[(f++) * pitch + c];
     }
 
+    // load / store of cell cb + cr with cb uniform across the wave and cr a
+    // 32-bit offset (cr * 8 < 2^32)
+    template<int D, int M>
+    static __device__ __forceinline__ void load_rel(const void* base, long long pitch, long long cb, unsigned cr,
+                                                    Cell<D, M>& U)
+    {
+        const double* q = (const double*)base + cb;
+        int f = 0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) U.m[k] = plane_at(q + (f++) * pitch, cr);
+#pragma unroll
+        for (int d = 0; d < D; ++d) U.mom[d] = plane_at(q + (f++) * pitch, cr);
+        U.E = plane_at(q + (f++) * pitch, cr);
+#pragma unroll
+        for (int k = 0; k < M - 1; ++k) U.a[k] = plane_at(q + (f++) * pitch, cr);
+    }
+
+    template<int D, int M>
+    static __device__ __forceinline__ void store_rel(void* base, long long pitch, long long cb, unsigned cr,
+                                                     const Cell<D, M>& U)
+    {
+        double* q = (double*)base + cb;
+        int f = 0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) plane_at(q + (f++) * pitch, cr) = U.m[k];
+#pragma unroll
+        for (int d = 0; d < D; ++d) plane_at(q + (f++) * pitch, cr) = U.mom[d];
+        plane_at(q + (f++) * pitch, cr) = U.E;
+#pragma unroll
+        for (int k = 0; k < M - 1; ++k) plane_at(q + (f++) * pitch, cr) = U.a[k];
+    }
+
     template<int D, int M>
     static __device__ __forceinline__ void store(void* base, long long pitch, long long c, const Cell<D, M>& U)
     {

@@ -117,6 +117,42 @@ __device__ __forceinline__ unsigned long long wave_reduce_u64(unsigned long long
     return ((unsigned long long)hi << 32) | lo;
 }
 
+// Max over a full wave of non-negative doubles (wave rates): DPP moves of the
+// two halves and an FP64 max per step, lane 63 ends with the result.  A NaN
+// rate never wins (IEEE maxNum), as a NaN candidate never wins the
+// reference's running std::min on dt.
+__device__ __forceinline__ double wave_max_f64(double v)
+{
+    const unsigned long long idle = 0ull;  // +0.0
+    const auto step = [&](unsigned long long o) { v = __builtin_fmax(v, bits_to_double(o)); };
+    step(dpp_u64<0xB1, 0xF>(double_to_bits(v), idle));   // quad_perm [1,0,3,2]
+    step(dpp_u64<0x4E, 0xF>(double_to_bits(v), idle));   // quad_perm [2,3,0,1]
+    step(dpp_u64<0x141, 0xF>(double_to_bits(v), idle));  // row_half_mirror
+    step(dpp_u64<0x140, 0xF>(double_to_bits(v), idle));  // row_mirror
+    step(dpp_u64<0x142, 0xA>(double_to_bits(v), idle));  // row_bcast:15 into rows 1, 3
+    step(dpp_u64<0x143, 0xC>(double_to_bits(v), idle));  // row_bcast:31 into rows 2, 3
+    const unsigned long long b = double_to_bits(v);
+    const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)b, 63);
+    const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+    return bits_to_double(((unsigned long long)hi << 32) | lo);
+}
+
+// the wave's combined CFL value as a bit pattern: rates by FP64 max,
+// candidates by the bit-ordered min
+template<typename Pol>
+__device__ __forceinline__ unsigned long long wave_cfl(double v)
+{
+    if constexpr (Pol::kCflRate) return double_to_bits(wave_max_f64(v));
+    else return wave_reduce_u64<false>(double_to_bits(v));
+}
+
+template<typename Pol>
+__device__ __forceinline__ unsigned long long cfl_combine_bits(unsigned long long a, unsigned long long b)
+{
+    if constexpr (Pol::kCflRate) return double_to_bits(__builtin_fmax(bits_to_double(a), bits_to_double(b)));
+    else return u64_combine<false>(a, b);
+}
+
 // CFL reduction of one block into its dt slots (bit-ordered atomicMin of a
 // positive double).  A policy contributes either dt candidates combined by
 // min (the FP64 form: min_cell min_d cfl dx_d/(|u_d|+c), cfl.hpp:43-63) or
@@ -133,7 +169,7 @@ __device__ __forceinline__ void cfl_block_atomic(double v, StepCtl* ctl, int par
     const int tid = threadIdx.x + threadIdx.y * blockDim.x + threadIdx.z * blockDim.x * blockDim.y;
     if constexpr (PER_WAVE) {
         constexpr int NW = NT / 64 > 0 ? NT / 64 : 1;
-        const unsigned long long w = wave_reduce_u64<Pol::kCflRate>(double_to_bits(v));
+        const unsigned long long w = wave_cfl<Pol>(v);
         if ((tid & 63) == 0) {
             unsigned long long* target = dt_slots(ctl, parity) + ((blk * NW + (tid >> 6)) % kDtSlots) * kDtStride;
             const double r = bits_to_double(w);
@@ -145,13 +181,13 @@ __device__ __forceinline__ void cfl_block_atomic(double v, StepCtl* ctl, int par
     unsigned long long* target = dt_slots(ctl, parity) + (blk % kDtSlots) * kDtStride;
     __shared__ unsigned long long red[NT / 64 > 0 ? NT / 64 : 1];
     __syncthreads();  // the block may reuse red[] for its next tile
-    const unsigned long long w = wave_reduce_u64<Pol::kCflRate>(double_to_bits(v));
+    const unsigned long long w = wave_cfl<Pol>(v);
     if ((tid & 63) == 0) red[tid >> 6] = w;
     __syncthreads();
     if (tid == 0) {
         unsigned long long m = red[0];
 #pragma unroll
-        for (int k = 1; k < NT / 64; ++k) m = u64_combine<Pol::kCflRate>(m, red[k]);
+        for (int k = 1; k < NT / 64; ++k) m = cfl_combine_bits<Pol>(m, red[k]);
         const double r = bits_to_double(m);
         const double dt = Pol::kCflRate ? (r > 0.0 ? 1.0 / r : bits_to_double(kInfBits)) : r;
         if (dt < bits_to_double(kInfBits)) atomicMin(target, double_to_bits(dt));
@@ -267,6 +303,28 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     idx[DIR] = i;
     const bool in = line_ok && i >= 0 && i < n;
     const long long cell = (long long)idx[0] + (long long)idx[1] * g.stride[1] + (long long)idx[2] * g.stride[2];
+    // x and y sweeps address the tile's cells as cbase (uniform across the
+    // block) + crel (a 32-bit offset below 2 * stride1 + L or L * stride1),
+    // so loads and stores take a scalar base and a 32-bit lane offset; z
+    // sweeps, whose tiles span L planes, keep 64-bit cell indices
+    constexpr bool REL = DIR < 2;
+    long long cbase = 0;
+    unsigned crel = 0;
+    if (DIR == 0) {
+        cbase = (long long)(bx * T - 3) + (long long)(by * W) * g.stride[1] + (long long)bz * g.stride[2];
+        crel = (unsigned)pos + (unsigned)lane * (unsigned)g.stride[1];
+    } else if (DIR == 1) {
+        cbase = (long long)(bx * W) + (long long)(by * T - 3) * g.stride[1] + (long long)bz * g.stride[2];
+        crel = (unsigned)lane + (unsigned)pos * (unsigned)g.stride[1];
+    }
+    const auto load_cell = [&](const void* q, Cell& U) {
+        if constexpr (REL) Pol::template load_rel<D, M>(q, g.pitch, cbase, crel, U);
+        else Pol::template load<D, M>(q, g.pitch, cell, U);
+    };
+    const auto store_cell = [&](void* q, const Cell& U) {
+        if constexpr (REL) Pol::template store_rel<D, M>(q, g.pitch, cbase, crel, U);
+        else Pol::template store<D, M>(q, g.pitch, cell, U);
+    };
     const bool end_cell = (i == 0) || (i == n - 1);
     const bool own = in && pos >= 3 && pos < L - 3;
 
@@ -279,8 +337,8 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
         // writing a buffer that already holds the state, need not.
         if (MODE == 0 && D % 2 == 1 && DIR == 0 && own) {
             Cell U;
-            Pol::template load<D, M>(qin, g.pitch, cell, U);
-            Pol::template store<D, M>(qout, g.pitch, cell, U);
+            load_cell(qin, U);
+            store_cell(qout, U);
         }
         return;
     }
@@ -291,7 +349,7 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     Pr Pn, Pl;
     R rhs_n[M];  // alpha_rhs_coefficients(P^n), used by the update
     if (in) {
-        Pol::template load<D, M>(qin, g.pitch, cell, Uc);
+        load_cell(qin, Uc);
         Pol::template to_state<D, M>(Uc, U);
         if constexpr (Pol::kStoredPressure) {
             if (pin) Pol::template cons_to_prim_stored<D, M>(P, U, pin[cell], Pn);
@@ -322,7 +380,7 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
             // the cell's state and P_line are read back (global, an L2 hit;
             // the thread's own LDS slot) instead of held across the barrier
             Cell U2;
-            Pol::template load<D, M>(qin, g.pitch, cell, U2);
+            load_cell(qin, U2);
             Pol::template to_state<D, M>(U2, U);
             get_prim<R, D, M>(sh, S, slot, 0, Pl);
         }
@@ -359,7 +417,7 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
             // than held in registers since phase 1
             if constexpr (Pol::kReloadInput) {
                 Cell Ue;
-                Pol::template load<D, M>(qin, g.pitch, cell, Ue);
+                load_cell(qin, Ue);
                 Pol::template to_state<D, M>(Ue, Ulh);
                 Pol::template cons_to_prim<D, M>(P, Ulh, PUl);
             } else {
@@ -434,7 +492,7 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
         // with Pol::kReloadInput the cell's input state is read again (an L2
         // hit) rather than held in registers across the four phases
         Cell Uin;
-        if constexpr (Pol::kReloadInput) Pol::template load<D, M>(qin, g.pitch, cell, Uin);
+        if constexpr (Pol::kReloadInput) load_cell(qin, Uin);
         else Uin = Uc;
         Cell r = Uin;
         if (n > 1) {
@@ -473,11 +531,11 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
         }
         if (MODE == 1) {
             Cell acc;
-            Pol::template load<D, M>(qout, g.pitch, cell, acc);
+            load_cell(qout, acc);
             Pol::template accumulate<D, M>(P, acc, r, Uin);
             r = acc;
         }
-        Pol::template store<D, M>(qout, g.pitch, cell, r);
+        store_cell(qout, r);
         if (FUSE_CFL) {
             bool interior = true;
 #pragma unroll
