@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -23,6 +24,40 @@ using namespace dimhip;
 namespace {
 
 thread_local std::string g_global_err;
+
+}  // namespace
+
+// ---- fixed scratch of dimhip_output_rows_noalloc ---------------------------
+// One array per GPU, held for the life of the process: the packed FP64 cells
+// (2 nmat + dim <= 9 doubles) and the CSV rows (3 + dim + 3 nmat <= 15
+// doubles) of at most kNoAllocMaxCells cells, 192 B per cell, 12 MiB.  Every
+// call on a GPU reuses it; each call writes every element it reads, so
+// nothing in it needs to start at zero; calls that overlap on one GPU race.
+constexpr long long kNoAllocMaxCells = 65536;
+constexpr size_t kNoAllocDoubles = (size_t)kNoAllocMaxCells * (9 + 15);
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of this GPU's scratch array (the array's name is a host
+// stand-in, not a device address): looked up once per device with the device
+// current, then reused.  Returns nonzero on failure.
+int scratch_address(int device, double** out)
+{
+    if (device < 0 || device >= kMaxDevices) return 1;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return 1;
+        g_scratch_ptr[device] = (double*)p;
+    }
+    *out = g_scratch_ptr[device];
+    return 0;
+}
 
 }  // namespace
 
@@ -871,6 +906,29 @@ int dimhip_output_rows(dimhip_ctx* c, double* rows, size_t ncells)
                                  hipMemcpyDeviceToHost, c->stream));
         HIP_OK(c, hipStreamSynchronize(c->stream));
     }
+    return 0;
+}
+
+int dimhip_output_rows_noalloc(dimhip_ctx* c, double* rows, size_t ncells)
+{
+    if (!c || !rows) return fail(c, "dimhip_output_rows_noalloc: null argument");
+    if ((long long)ncells != c->grid.ncells) return fail(c, "dim::write_csv: grid size mismatch");
+    if (ncells > (size_t)kNoAllocMaxCells)
+        return fail(c, "dimhip_output_rows_noalloc: " + std::to_string(ncells) + " cells exceed the " +
+                           std::to_string(kNoAllocMaxCells) + " of its fixed scratch; use dimhip_output_rows");
+    HIP_OK(c, hipSetDevice(c->desc.device));
+    double* base = nullptr;
+    if (scratch_address(c->desc.device, &base)) return fail(c, "dimhip_output_rows_noalloc: hipGetSymbolAddress failed");
+    const int nr = 3 + c->desc.dim + 3 * c->desc.nmat;
+    const long long cnt = c->grid.ncells;
+    // the packed FP64 cells, then the rows, both inside the scratch array
+    double* staging = base;
+    double* out = base + (size_t)cnt * c->nv;
+    c->ops->pack(c, false, staging, c->q[c->cur], 0, cnt);
+    c->ops->output_rows(c, staging, out, cnt);
+    HIP_OK(c, hipGetLastError());
+    HIP_OK(c, hipMemcpyAsync(rows, out, sizeof(double) * (size_t)(cnt * nr), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 

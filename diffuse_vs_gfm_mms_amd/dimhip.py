@@ -161,6 +161,7 @@ def load_library():
         "dimhip_upload_primitive": (C.c_int, [vp, dp, C.c_size_t]),
         "dimhip_initialise": (C.c_int, [vp, C.POINTER(Ic)]),
         "dimhip_output_rows": (C.c_int, [vp, dp, C.c_size_t]),
+        "dimhip_output_rows_noalloc": (C.c_int, [vp, dp, C.c_size_t]),
         "dimhip_synchronize": (C.c_int, [vp]),
         "dimhip_set_timing": (C.c_int, [vp, C.c_int]),
         "dimhip_kernel_times": (C.c_int, [vp, dp, llp]),
@@ -191,7 +192,7 @@ def exported_symbols():
     return ["dimhip_version", "dimhip_create", "dimhip_destroy", "dimhip_upload", "dimhip_download",
             "dimhip_step", "dimhip_advance", "dimhip_compute_dt", "dimhip_sweep", "dimhip_apply_bc",
             "dimhip_conservation", "dimhip_boundary_flux", "dimhip_track_boundary_flux",
-            "dimhip_boundary_flux_integral", "dimhip_upload_primitive", "dimhip_initialise", "dimhip_output_rows", "dimhip_synchronize", "dimhip_set_timing", "dimhip_kernel_times", "dimhip_sweep_times",
+            "dimhip_boundary_flux_integral", "dimhip_upload_primitive", "dimhip_initialise", "dimhip_output_rows", "dimhip_output_rows_noalloc", "dimhip_synchronize", "dimhip_set_timing", "dimhip_kernel_times", "dimhip_sweep_times",
             "dimhip_kat", "dimhip_last_error", "dimhip_batch_create", "dimhip_batch_destroy",
             "dimhip_batch_upload", "dimhip_batch_download", "dimhip_batch_configure", "dimhip_batch_autotune",
             "dimhip_batch_advance", "dimhip_batch_set_timing", "dimhip_batch_kernel_times", "dimhip_batch_sweep_times",

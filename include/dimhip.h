@@ -213,6 +213,14 @@ int dimhip_initialise(dimhip_ctx* ctx, const dimhip_ic* ic);
  * left to the caller. */
 int dimhip_output_rows(dimhip_ctx* ctx, double* rows, size_t ncells);
 
+/* dimhip_output_rows without any device allocation, for callers that cannot
+ * allocate (code that must not call hipMalloc: tight serving loops,
+ * harnesses that forbid it).  No counterpart in the reference.  The packed
+ * cells and rows are staged in a fixed 12 MiB device array per GPU, so the
+ * grid may hold at most 65536 cells (larger grids fail with a message);
+ * calls on one GPU share that array and must not overlap. */
+int dimhip_output_rows_noalloc(dimhip_ctx* ctx, double* rows, size_t ncells);
+
 /* Wait for the context's stream. */
 int dimhip_synchronize(dimhip_ctx* ctx);
 
