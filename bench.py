@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=1000,
                     help="steps of the batched 64-instance companion measurement (0: skip)")
     ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--batch-tuning", default=os.path.join(ROOT, "diffuse_vs_gfm_mms_amd", "batch_tuning.json"),
+                    help="JSON file of stored autotuner choices (read, and written when the instance set has none)")
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="steps per run of the reference CPU solver on helium 2048x1024 (0: the sample's default, "
                          "2; 1 warm-up + 3 timed runs, about 15 s)")
@@ -479,13 +481,16 @@ def measure_tube(dimhip, workloads, cpu):
     return out
 
 
-def measure_batch(dimhip, workloads, count, steps):
+def measure_batch(dimhip, workloads, count, steps, tuning):
     """Seeded uneven shock-interface instances, FP32-mixed, advanced together
-    (autotuned tiles and schedule, hipGraph-captured steps)."""
+    (autotuned tiles and schedule, hipGraph-captured steps).  The autotuner's
+    choice for this instance set is read from the JSON file `tuning` when it
+    holds one; otherwise every configuration is timed twice over 10 steps and
+    the choice stored there."""
     cases = workloads.batch_instances(count=count)
     cells = sum(c.ncells for c in cases)
     with dimhip.Batch(cases) as b:
-        tune = b.autotune(0.8, trial_steps=3)
+        tune = b.autotune(0.8, trial_steps=10, repeats=2, cache=tuning)
         b.advance(10, 0.8, use_graph=True)
         t0 = time.perf_counter()
         _, nsteps = b.advance(steps, 0.8, use_graph=True)
@@ -508,7 +513,8 @@ def measure_batch(dimhip, workloads, count, steps):
         "unit": "cell-updates/s",
         "cells": cells,
         "ms_per_step": wall / steps * 1e3,
-        "autotuned": tune["best"],
+        "autotuned": {"best": tune["best"], "cached": tune["cached"], "key": tune["key"],
+                      "trial_steps": tune["trial_steps"], "repeats": tune["repeats"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "kernel": f"batched {AXES[dom]} sweep (the slower of the two persistent sweep launches)",
@@ -625,7 +631,7 @@ def main():
             out["fp32_mixed"]["accuracy_vs_fp64"] = accuracy_fp32(dimhip, workloads, g32, acc_steps)
     if args.batch_steps > 0 and precision == "fp64":
         progress(f"batched {args.batch_count} instances: {args.batch_steps} steps")
-        out["batched"] = measure_batch(dimhip, workloads, args.batch_count, args.batch_steps)
+        out["batched"] = measure_batch(dimhip, workloads, args.batch_count, args.batch_steps, args.batch_tuning)
         if args.accuracy_steps != 0:
             progress(f"batched instances 0..3 fp32_mixed vs fp64 accuracy: {args.batch_steps} steps")
             out["batched"]["accuracy_vs_fp64"] = accuracy_batch(dimhip, workloads, args.batch_count,

@@ -393,6 +393,42 @@ def kat(which, dim, materials, count, in0, in1=None, d=0, lambda_model="kapila",
     return out0[:size0], out1[:count]
 
 
+def batch_tuning_key(cases):
+    """Identity of a batched instance set for the autotuner cache: every
+    instance's grid, spacing, materials, boundaries and lambda model, and the
+    ABI version (a new library may rank the configurations differently)."""
+    import hashlib
+    import json
+    desc = [[list(c.N), [float(x) for x in c.dx], [list(m) for m in c.materials], list(c.bc_lo), list(c.bc_hi),
+             c.lambda_model, float(c.alpha_source_floor)] for c in cases]
+    blob = json.dumps({"abi": load_library().dimhip_version(), "instances": desc}, sort_keys=True)
+    return hashlib.sha1(blob.encode()).hexdigest()[:16]
+
+
+def load_tuned(path, key):
+    """The stored autotuner choice for `key`, or None."""
+    import json
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get(key)
+
+
+def store_tuned(path, key, entry):
+    """Store an autotuner choice under `key` (other keys are kept)."""
+    import json
+    data = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            data = json.load(f)
+    data[key] = entry
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+        f.write("\n")
+    os.replace(tmp, path)
+
+
 class Batch:
     """Many 2D two-material FP32-mixed instances advanced together
     (include/dimhip.h, batched form)."""
@@ -445,11 +481,39 @@ class Batch:
     def configure(self, x_variant, y_variant, schedule, blocks_per_cu):
         self._check(self.lib.dimhip_batch_configure(self.h, x_variant, y_variant, schedule, blocks_per_cu))
 
-    def autotune(self, cfl, trial_steps=3):
+    def autotune(self, cfl, trial_steps=3, repeats=1, cache=None):
+        """Choose the tile shapes, schedule and blocks per CU.  With `cache` (a
+        JSON file path) a choice stored for this instance set is applied
+        without timing; otherwise the device autotuner times every
+        configuration `repeats` times over `trial_steps` steps (each time from
+        the current state, which it restores), the configuration with the
+        smallest of its times is applied, and with `cache` it is stored."""
         import json
-        buf = C.create_string_buffer(16384)
-        self._check(self.lib.dimhip_batch_autotune(self.h, cfl, trial_steps, buf, len(buf)))
-        return json.loads(buf.value.decode())
+        key = batch_tuning_key(self.cases)
+        if cache is not None:
+            hit = load_tuned(cache, key)
+            if hit is not None:
+                b = hit["best"]
+                self.configure(b["x"], b["y"], b["schedule"], b["blocks_per_cu"])
+                return dict(hit, cached=True, key=key)
+        best_ms = {}
+        report = None
+        for _ in range(max(1, repeats)):
+            buf = C.create_string_buffer(16384)
+            self._check(self.lib.dimhip_batch_autotune(self.h, cfl, trial_steps, buf, len(buf)))
+            report = json.loads(buf.value.decode())
+            for t in report["trials"]:
+                k = (t["x"], t["y"], t["schedule"], t["blocks_per_cu"])
+                best_ms[k] = min(best_ms.get(k, float("inf")), t["ms"])
+        k = min(best_ms, key=best_ms.get)
+        self.configure(*k)
+        report["best"] = {"x": k[0], "y": k[1], "schedule": k[2], "blocks_per_cu": k[3], "ms": best_ms[k]}
+        report.update(trial_steps=trial_steps, repeats=repeats, cached=False, key=key)
+        if cache is not None:
+            store_tuned(cache, key, {"best": report["best"], "trial_steps": trial_steps, "repeats": repeats,
+                                     "instances": len(self.cases),
+                                     "cells": int(sum(c.ncells for c in self.cases))})
+        return report
 
     def advance(self, nsteps, cfl, t_end=1e300, use_graph=True):
         steps = (C.c_longlong * len(self.cases))()

@@ -66,3 +66,19 @@ def test_dominant_sweep_roofline():
     assert r["kernel"].startswith("y sweep")
     assert abs(r["achieved"] - b / 0.6e-3 / 1e9) < 1e-9
     assert r["frac"] == r["per_axis"]["y"]["frac"] < r["per_axis"]["x"]["frac"]
+
+
+def test_batch_tuning_cache(tmp_path):
+    """The autotuner cache is keyed by the instance set and keeps other keys."""
+    from diffuse_vs_gfm_mms_amd import dimhip
+    a = W.batch_instances(count=3)
+    b = W.batch_instances(count=4)
+    ka, kb = dimhip.batch_tuning_key(a), dimhip.batch_tuning_key(b)
+    assert ka == dimhip.batch_tuning_key(W.batch_instances(count=3)) and ka != kb
+    path = str(tmp_path / "tune.json")
+    assert dimhip.load_tuned(path, ka) is None
+    entry = {"best": {"x": 1, "y": 2, "schedule": 0, "blocks_per_cu": 4, "ms": 0.5}}
+    dimhip.store_tuned(path, ka, entry)
+    dimhip.store_tuned(path, kb, dict(entry, best=dict(entry["best"], x=0)))
+    assert dimhip.load_tuned(path, ka) == entry
+    assert dimhip.load_tuned(path, kb)["best"]["x"] == 0
