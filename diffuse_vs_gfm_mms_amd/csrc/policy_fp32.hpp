@@ -538,8 +538,18 @@ __device__ __forceinline__ void reconstruct(const Pr<D, M>& L, const Pr<D, M>& C
     const float s = 0.5f * minmod(C.p - L.p, R.p - C.p);
     lf.p = C.p - s;
     rf.p = C.p + s;
-    enforce<D, M>(lf);
-    enforce<D, M>(rf);
+    // enforce_positive_primitive (muscl.hpp:26-40).  A MinBee face
+    // C -/+ s/2 with s = fl(C-L) or fl(R-C) (the smaller, same sign) or 0
+    // lies between its neighbours' values even after rounding, and the
+    // inputs are floored primitives (densities >= 1e-12, fractions >= 0,
+    // p >= 1e-12), so the floors cannot act.  For two materials the faces'
+    // fractions sum to one to FP32 rounding (the two slopes are each other's
+    // negatives), and the renormalisation only moves them by that rounding;
+    // both are left out.  Three or more materials keep the full enforce.
+    if constexpr (M > 2) {
+        enforce<D, M>(lf);
+        enforce<D, M>(rf);
+    }
 }
 
 // sum_k alpha_k / (rho_k c_k^2), the Wood compressibility (eos.hpp:195-221),
@@ -1132,7 +1142,7 @@ struct Fp32MixedT {
 #pragma unroll
         for (int k = 0; k < M; ++k) an[k] = Pn.alpha[k] - lf * (afr[k] - afl[k]) + dtf * rhs[k] * div_u;
         f32::sanitise<M>(an, 0.f);
-        if (P.stiff) {
+        if (law_stiff(P)) {
             float ad_old[M], ad_new[M];
             f32::decode_alpha<M>(r.a, ad_old);
             f32::encode_alpha<M>(an, r.a);
