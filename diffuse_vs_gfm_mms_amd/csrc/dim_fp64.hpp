@@ -206,6 +206,66 @@ __device__ __forceinline__ double mix_energy(const EP& P, const MixCoef<M, NA>& 
     return finite_or(s);
 }
 
+// The certain halvings of the bisection, as one wave-uniform loop of at most
+// 80 passes.  On entry `go` (a subset of the active lanes) holds the lanes
+// that may fast-forward; pass p (p = 0, 1, ...) forms mid = 0.5 * (lo + hi),
+// the reference's midpoint, on every lane still going, and a lane whose mid
+// lies above p_up takes hi = mid, below p_dn lo = mid; a lane whose mid lies
+// in [p_dn, p_up] stops with n = p (the halvings it took), and a lane still
+// going after pass 79 ends with n = 80.  Lanes outside `go` keep lo, hi, n.
+// The per-lane choice is made with the execution mask (64-bit moves under
+// the lanes that go up, then those that go down), so a pass costs two FP64
+// operations, two compares and two moves.  The loop runs on scalar registers
+// only (counter, masks), touches no memory and restores the execution mask.
+__device__ __forceinline__ void fast_forward_halvings(double& lo, double& hi, int& n, double p_up, double p_dn,
+                                                      unsigned long long go)
+{
+    double mid;
+    unsigned long long save, up, dn, t1, t2, t3;
+    unsigned cnt;
+    asm volatile(
+        "s_mov_b64 %[save], exec\n\t"
+        "s_and_b64 exec, exec, %[go]\n\t"
+        "s_mov_b32 %[cnt], 0\n\t"
+        "s_cbranch_execz .Lff_end_%=\n"
+        ".Lff_loop_%=:\n\t"
+        "v_add_f64 %[mid], %[lo], %[hi]\n\t"
+        "v_mul_f64 %[mid], %[mid], 0.5\n\t"
+        "v_cmp_gt_f64_e64 %[up], %[mid], %[pup]\n\t"
+        "v_cmp_lt_f64_e64 %[dn], %[mid], %[pdn]\n\t"
+        "s_or_b64 %[t1], %[up], %[dn]\n\t"
+        "s_andn2_b64 %[t2], exec, %[t1]\n\t"
+        "s_cmp_eq_u64 %[t2], 0\n\t"
+        "s_cbranch_scc1 .Lff_keep_%=\n\t"
+        "s_mov_b64 %[t3], exec\n\t"
+        "s_mov_b64 exec, %[t2]\n\t"
+        "v_mov_b32 %[n], %[cnt]\n\t"
+        "s_mov_b64 exec, %[t3]\n"
+        ".Lff_keep_%=:\n\t"
+        "s_and_b64 exec, exec, %[t1]\n\t"
+        "s_and_b64 %[t2], exec, %[up]\n\t"
+        "s_andn2_b64 %[t3], exec, %[up]\n\t"
+        "s_mov_b64 %[t1], exec\n\t"
+        "s_mov_b64 exec, %[t2]\n\t"
+        "v_mov_b64 %[hi], %[mid]\n\t"
+        "s_mov_b64 exec, %[t3]\n\t"
+        "v_mov_b64 %[lo], %[mid]\n\t"
+        "s_mov_b64 exec, %[t1]\n\t"
+        "s_add_u32 %[cnt], %[cnt], 1\n\t"
+        "s_cmp_eq_u32 %[cnt], 80\n\t"
+        "s_cbranch_scc1 .Lff_full_%=\n\t"
+        "s_cbranch_execnz .Lff_loop_%=\n\t"
+        "s_branch .Lff_end_%=\n"
+        ".Lff_full_%=:\n\t"
+        "v_mov_b32 %[n], 80\n"
+        ".Lff_end_%=:\n\t"
+        "s_mov_b64 exec, %[save]"
+        : [lo] "+v"(lo), [hi] "+v"(hi), [n] "+v"(n), [mid] "=&v"(mid), [save] "=&s"(save), [up] "=&s"(up),
+          [dn] "=&s"(dn), [t1] "=&s"(t1), [t2] "=&s"(t2), [t3] "=&s"(t3), [cnt] "=&s"(cnt)
+        : [go] "s"(go), [pup] "v"(p_up), [pdn] "v"(p_dn)
+        : "scc");
+}
+
 // eos.hpp:95-142: bracket doubling from [1e-12, max(1, fallback)], then
 // bisection to |f| <= 1e-10 max(1, rho e)
 template<int M, bool NA, typename EP>
@@ -254,25 +314,35 @@ __device__ __forceinline__ double bisect_pressure(const EP& P, double target, do
         const double Cmt = C - target;
         const double B = Sabs * smax(hi, 1e-10) + Cabs + __builtin_fabs(target);  // pk <= max(hi, 1e-10) from here on
         const double thr = tol + 3.5527136788005009e-15 * B;                      // 32 u B
-        // The halvings run as a wave-uniform loop: each lane's updates are
-        // predicated on its own `go`, which falls permanently at its first
-        // uncertain halving, and n counts the halvings the lane took (its loop
-        // index on leaving the per-lane loop this replaces).  A lane takes at
-        // most one halving per pass, so the pass counter bounds n by 80.
-        // lo and hi are finite here (hi passed the finite(fhi) test), so mid
-        // is finite and fmax(mid, 1e-10) is smax(mid, 1e-10); a lane still
-        // going at pass p has taken p + 1 halvings.
-        bool go = finite(S) && finite(Cmt) && B < 1e290;
-        for (int pass = 0; pass < 80; ++pass) {
-            if (__builtin_amdgcn_ballot_w64(go) == 0) break;
-            const double mid = 0.5 * (lo + hi);
-            const double e = __builtin_fma(S, __builtin_fmax(mid, 1e-10), Cmt);
-            go = go && __builtin_fabs(e) > thr;
-            const bool up = e > 0.0;
-            hi = (go && up) ? mid : hi;
-            lo = (go && !up) ? mid : lo;
-            n = go ? pass + 1 : n;
-        }
+        // The halvings run as a wave-uniform loop (fast_forward_halvings):
+        // each lane goes on until its first uncertain halving, and n counts the
+        // halvings it took (its loop index on leaving the per-lane loop this
+        // replaces).
+        //
+        // The test |e| > thr is taken on mid itself.  With S~ > 0 and L(p) =
+        // S~ p + Cmt in exact arithmetic, e = RN(L(pk)), so L(pk) > T =
+        // RN(thr (1 + 4u)) >= thr (1 + 2u) gives e >= L (1 - u) > thr, and
+        // L(pk) < -T gives e < -thr.  L(pk) > T holds for pk > q_up = (T -
+        // Cmt) / S~ and L(pk) < -T for pk < q_dn = (-T - Cmt) / S~.  Their
+        // rounded values (one reciprocal of S~, a subtraction and a product)
+        // carry a relative error below 3.01 u; moved outward by 2^-50 = 8u of
+        // their magnitude (p_up up, p_dn down, either sign, the move's own
+        // rounding included) they bound the exact ones.  lo and hi are finite
+        // here (hi passed the finite(fhi) test) and lo >= 1e-12, so mid is
+        // finite and positive, and pk = max(mid, 1e-10) >= mid: mid > p_up is
+        // a certain upward halving; mid < p_dn is a certain downward one when
+        // p_dn > 1e-10 (then pk < p_dn whichever of mid and 1e-10 it is), and
+        // is never taken otherwise (p_dn = 0).  p_dn < p_up, and every mid with
+        // |e| <= thr lies in [p_dn, p_up], so a lane stops at or before the
+        // halving where the test on e would stop it.
+        const double T = thr * 1.0000000000000004;  // 1 + 2^-51
+        const double rS = 1.0 / S;
+        const double q_up = (T - Cmt) * rS, q_dn = (-T - Cmt) * rS;
+        const double p_up = q_up + __builtin_fabs(q_up) * 8.8817841970012523e-16;  // 2^-50
+        double p_dn = q_dn - __builtin_fabs(q_dn) * 8.8817841970012523e-16;
+        p_dn = (p_dn > 1e-10) ? p_dn : 0.0;
+        const bool go = finite(S) && S > 0.0 && finite(Cmt) && B < 1e290 && finite(p_up) && finite(p_dn);
+        fast_forward_halvings(lo, hi, n, p_up, p_dn, __builtin_amdgcn_ballot_w64(go));
     }
     for (; n < 80; ++n) {
         const double mid = 0.5 * (lo + hi);
@@ -411,8 +481,13 @@ __device__ __forceinline__ void sanitise_alpha(double* alpha, double floor)
         for (int k = 0; k < M; ++k) alpha[k] = (k == 0) ? 1.0 : 0.0;
         return;
     }
+    // x / 1.0 is x for every finite x, and the fractions are finite here: the
+    // M divisions run only where the sum differs from one (for two materials
+    // stored as alpha_0 and 1 - alpha_0 the rounded sum is exactly one)
+    if (sum != 1.0) {
 #pragma unroll
-    for (int k = 0; k < M; ++k) alpha[k] /= sum;
+        for (int k = 0; k < M; ++k) alpha[k] /= sum;
+    }
 }
 
 // primitives.hpp:99-108
@@ -567,14 +642,9 @@ __device__ __forceinline__ void repair_state(const EP& P, State<D, M>& U)
     for (int d = 0; d < D; ++d) U.mom[d] = finite_or(U.mom[d]);
     U.E = finite_or(U.E);
 
+    // rho_before, vel_before and v2_before are read only where inactive mass
+    // is removed, and are formed there from the same (unchanged) momenta
     const double rho_before = smax(total_density<D, M>(U), mass_floor);
-    double vel_before[D];
-    double v2_before = 0.0;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        vel_before[d] = safe_div(U.mom[d], rho_before);
-        v2_before += vel_before[d] * vel_before[d];
-    }
     // alpha_before equals alpha_full below: both are sanitise(full_alpha(U.alpha), 0)
     // of the same (already finite_or'd) values
     double alpha_full[M];
@@ -605,6 +675,13 @@ __device__ __forceinline__ void repair_state(const EP& P, State<D, M>& U)
         rho_total = mass_floor;
     }
     if (removed) {
+        double vel_before[D];
+        double v2_before = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            vel_before[d] = safe_div(U.mom[d], rho_before);
+            v2_before += vel_before[d] * vel_before[d];
+        }
         double rho_mb[M];
 #pragma unroll
         for (int k = 0; k < M; ++k)
@@ -623,18 +700,47 @@ __device__ __forceinline__ void repair_state(const EP& P, State<D, M>& U)
         const double ie = internal_energy_from_pressure<M>(P, rho_total, pressure_before, alpha_full, rho_m);
         U.E = rho_total * ie + 0.5 * rho_total * v2_before;
     }
-    double v2 = 0.0;
+    // The energy floor U.E = max(U.E, kinetic + minimum_internal) leaves U.E
+    // unchanged where U.E exceeds the floor's rounded value R, which is tested
+    // first without its divisions.  With rho = rho_total in [1e-10, 1e300)
+    // (safe_div's denominator is then rho itself), the reference's kinetic =
+    // RN(RN(0.5 rho) v2), v2 the rounded sum of RN(RN(mom_d / rho)^2), is at
+    // most 0.5 sum_d mom_d^2 / rho (1 + 8.01u) for D <= 3, and each term
+    // RN(coef_k RN(num_k / den_k)) of minimum_internal at most alpha_k
+    // emin_ub[k] (1 + 5.01u) (coef = alpha rho_k, den = (gamma-1) rho_k or its
+    // 1e-10 guard, which only lowers the term); with the M <= 4 additions and
+    // the final one R <= (0.5 sum mom^2 / rho + sum_k alpha_k emin_ub[k])
+    // (1 + 14u).  Below, rhs >= (0.5 m2 + rho a)(1 + 2^-40)(1 - 7.01u) with
+    // m2 = sum mom^2 and a = sum alpha emin_ub exact (at most max(D, M + 1) +
+    // 2 roundings, each lowering it by at most u), and lhs <= rho U.E (1 + u),
+    // so lhs > rhs gives U.E > (0.5 m2 / rho + a)(1 + 2^-40 - 8.02u) > R
+    // (2^-40 = 8192u): the floor is skipped with the same result.  Otherwise
+    // (U.E <= 0, or a non-finite term) the floor is evaluated.
+    bool floor_needed = true;
+    if (rho_total >= 1e-10 && rho_total < 1e300) {
+        double m2 = 0.0, a = 0.0;
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-        const double v = safe_div(U.mom[d], rho_total);
-        v2 += v * v;
+        for (int d = 0; d < D; ++d) m2 += U.mom[d] * U.mom[d];
+#pragma unroll
+        for (int k = 0; k < M; ++k) a += alpha_full[k] * P.emin_ub[k];
+        const double rhs = (0.5 * m2 + rho_total * a) * 1.0000000000009095;  // 1 + 2^-40
+        const double lhs = U.E * rho_total;
+        floor_needed = !(lhs > rhs && lhs < 1.7e308);
     }
-    double rho_m[M];
+    if (floor_needed) {
+        double v2 = 0.0;
 #pragma unroll
-    for (int k = 0; k < M; ++k) rho_m[k] = smax(safe_div(U.m[k], smax(alpha_full[k], inactive)), mass_floor);
-    const double kinetic = 0.5 * rho_total * v2;
-    const double minimum_internal = mixture_internal_energy_density<M>(P, p_floor, alpha_full, rho_m);
-    U.E = smax(finite_or(U.E), kinetic + minimum_internal);
+        for (int d = 0; d < D; ++d) {
+            const double v = safe_div(U.mom[d], rho_total);
+            v2 += v * v;
+        }
+        double rho_m[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) rho_m[k] = smax(safe_div(U.m[k], smax(alpha_full[k], inactive)), mass_floor);
+        const double kinetic = 0.5 * rho_total * v2;
+        const double minimum_internal = mixture_internal_energy_density<M>(P, p_floor, alpha_full, rho_m);
+        U.E = smax(finite_or(U.E), kinetic + minimum_internal);
+    }
 }
 
 // ---- reconstruction (src/core/fv/limiters.hpp, src/dim/reconstruction/muscl.hpp)

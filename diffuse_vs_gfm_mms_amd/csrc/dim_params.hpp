@@ -20,6 +20,13 @@ struct EosParams {
     double gm1[kMaxMat];
     double gpinf[kMaxMat];
     double covol[kMaxMat];
+    // an upper bound of the minimum internal energy density per unit volume
+    // fraction, e_k(p = 1e-10) rho_k: (1e-10 + gamma p_inf) / (gamma - 1) for a
+    // stiffened gas and 1e-10 / (gamma - 1) otherwise (noble-Abel: co-volume
+    // >= 0), raised by a relative 1e-13 over the rounding of the host's
+    // division; +inf where no such bound holds.  repair_state uses it to skip
+    // an energy floor that cannot act.
+    double emin_ub[kMaxMat];
     int noble_abel;  // nonzero when any material is noble-Abel
     int lambda_model;
     double alpha_source_floor;

@@ -226,6 +226,10 @@ void fill_eos(EosParams& P, const dimhip_desc& d)
         P.covol[k] = (on && P.kind[k] == EOS_NOBLE_ABEL) ? d.covolume[k] : 0.0;
         P.gm1[k] = P.gamma[k] - 1.0;
         P.gpinf[k] = P.gamma[k] * P.pinf[k];
+        const double num = P.kind[k] == EOS_STIFFENED ? 1e-10 + P.gpinf[k] : 1e-10;
+        const bool bounded = P.gm1[k] > 0.0 && P.gpinf[k] >= 0.0 && P.covol[k] >= 0.0;
+        P.emin_ub[k] = bounded ? num / P.gm1[k] * (1.0 + 1e-13) : HUGE_VAL;
+        if (!(P.emin_ub[k] < HUGE_VAL)) P.emin_ub[k] = HUGE_VAL;
         if (on && P.kind[k] == EOS_NOBLE_ABEL) P.noble_abel = 1;
     }
     P.lambda_model = d.lambda_model;
