@@ -43,8 +43,10 @@ __device__ __forceinline__ void work_range(int nwork, int schedule, int& beg, in
     }
 }
 
+// the single-grid sweep's register cap (8 waves per SIMD for the FP32-mixed
+// two-material tiles), which the persistent loop's few extra registers fit
 template<typename Pol, int DIR, int L, int W, bool FUSE>
-__global__ void __launch_bounds__(L* W)
+__global__ void __launch_bounds__(L* W) __attribute__((amdgpu_waves_per_eu(Pol::template min_waves_per_eu<2, 2>())))
 batch_sweep_kernel(const BatchInst<Pol>* __restrict__ insts, const Work* __restrict__ work, int nwork, int schedule,
                    int cur)
 {

@@ -238,9 +238,7 @@ __device__ __forceinline__ void xcd_block(int& bx, int& by, int& bz)
 // of repeating the bisection.  pout (with FUSE_CFL): the recovered pressure
 // of every interior output cell, for the next sweep.
 // PERSISTENT: the block runs many tiles (the batched launches).  Its CFL
-// epilogue then reduces per wave (cfl_block_atomic's PER_WAVE), and no LDS
-// rows are added for Pol::kParkAlpha, since the batched autotuner packs
-// several blocks per CU.
+// epilogue then reduces per wave (cfl_block_atomic's PER_WAVE).
 // ---------------------------------------------------------------------------
 template<typename Pol, int D, int M, int DIR, int L, int W, int MODE, bool FUSE_CFL, bool PERSISTENT = false>
 __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* __restrict__ qout, const Grid& g,
@@ -260,7 +258,7 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     // Pol::kParkAlpha: P^n's fractions and rhs_n wait in NPARK extra LDS rows
     // of the thread's own slot from phase 1 to phase 5 (for nmat <= 2, whose
     // tiles still fit the occupancy the register cap allows)
-    constexpr bool PARK = Pol::kParkAlpha && M <= 2 && !PERSISTENT;
+    constexpr bool PARK = Pol::kParkAlpha && M <= 2;
     constexpr int NPARK = PARK ? 2 * M : 0;
     __shared__ R sh[(NREC + NPARK) * S];
 
