@@ -22,6 +22,14 @@
 
 #include "policy_fp64.hpp"
 
+// Products and sums within one expression are fused into FMAs (llvm.fmuladd)
+// in this policy's code only: it has no bit-parity contract, and a fused
+// a*b+c is one instruction and one rounding.  The decision is made per
+// expression at parse time, so every inlining context (single-grid and
+// batched sweeps) computes the same bits.  The FP64 policy, compiled with
+// -ffp-contract=off, is outside this region.
+#pragma clang fp contract(on)
+
 namespace dimhip {
 
 struct EosMixed {
@@ -187,14 +195,14 @@ template<int M>
 __device__ __forceinline__ void decode_alpha(const float* a, float* alpha)
 {
     if constexpr (M == 2) {
-        // the sign bit selects the branch: -0.0 encodes alpha_1 = 0
-        if (!__builtin_signbit(a[0])) {
-            alpha[0] = a[0];
-            alpha[1] = 1.f - a[0];
-        } else {
-            alpha[1] = -a[0];
-            alpha[0] = 1.f + a[0];
-        }
+        // the smaller fraction is |a| and the larger 1 - |a| (1 + a for a
+        // negative a, the same rounding); the sign bit says which one is
+        // alpha_0, so -0.0 encodes alpha_1 = 0
+        const float small = __builtin_fabsf(a[0]);
+        const float large = 1.f - small;
+        const bool neg = __builtin_signbit(a[0]);
+        alpha[0] = neg ? large : small;
+        alpha[1] = neg ? small : large;
     } else {
         float sum = -0.f;
 #pragma unroll
@@ -270,6 +278,9 @@ __device__ __forceinline__ bool valid_code(const float* a)
 template<int M>
 __device__ __forceinline__ bool interface_cell(const float* alpha)
 {
+    // two materials: the fractions sum to one, so both lie in
+    // (1e-6, 1-1e-6) exactly when the smaller one exceeds 1e-6
+    if constexpr (M == 2) return fmins(alpha[0], alpha[1]) > 1e-6f;
     bool itf = false;
 #pragma unroll
     for (int k = 0; k < M; ++k) itf |= (alpha[k] > 1e-6f && alpha[k] < 1.f - 1e-6f);
@@ -278,26 +289,30 @@ __device__ __forceinline__ bool interface_cell(const float* alpha)
 
 // p from the shifted internal energy density
 // (rho e - B(alpha)) = sum_k alpha_k w_k p/(gamma_k-1) - sum_k alpha_k (1 - w_k) b_k,
-// coef[k] = ecoef of the material's partial density
+// coef[k] = ecoef of the material's partial density.  The fractions are
+// non-negative (floored by cons_to_prim, sanitised or decoded from a valid
+// code by repair).
 template<int M, typename EP>
 __device__ __forceinline__ float pressure_shifted(const EP& P, float rho_e_hat, const float* alpha,
                                                   const float* coef)
 {
     float S = -0.f;
 #pragma unroll
-    for (int k = 0; k < M; ++k) S += fmaxs(alpha[k], 0.f) * coef[k];
+    for (int k = 0; k < M; ++k) S += alpha[k] * coef[k];
     if (law_stiff_ghost(P)) rho_e_hat += ghost_shift<M>(P, alpha, coef);
     return rho_e_hat * rcp(S);
 }
 
-// rho e - B at pressure p: sum_k alpha_k w_k max(p,1e-10)/(gamma_k-1) - sum_k alpha_k (1 - w_k) b_k
+// rho e - B at pressure p: sum_k alpha_k w_k max(p,1e-10)/(gamma_k-1) - sum_k alpha_k (1 - w_k) b_k,
+// for p >= 1e-10 and non-negative fractions (its callers' faces lie between
+// cons_to_prim outputs, whose pressure is floored at 1e-10; repair's
+// rebuild floors its pressure itself)
 template<int M, typename EP>
 __device__ __forceinline__ float energy_shifted(const EP& P, float p, const float* alpha, const float* coef)
 {
-    const float pk = fmaxs(p, 1e-10f);
     float s = -0.f;
 #pragma unroll
-    for (int k = 0; k < M; ++k) s += fmaxs(alpha[k], 0.f) * (pk * coef[k]);
+    for (int k = 0; k < M; ++k) s += alpha[k] * (p * coef[k]);
     if (law_stiff_ghost(P)) s -= ghost_shift<M>(P, alpha, coef);
     return s;
 }
@@ -346,8 +361,14 @@ __device__ __forceinline__ void cons_to_prim(const EP& P, const St<D, M>& U, Pr<
     }
     const float kinetic = 0.5f * rho * v2;
     const float shift = law_stiff(P) ? bshift<M>(P, ad, R.alpha) : 0.f;
+    // Stiffened materials: the shifted energy of a fast, low-pressure liquid
+    // is mostly kinetic (water at 480 m/s and 3e4 Pa: 1 part in 1e4), so
+    // there the kinetic energy is subtracted in FP64 too, whatever the
+    // fractions; otherwise cells next to the interface-cell threshold would
+    // alternate between the two recoveries with rounding noise.
+    const bool fast_liquid = law_stiff(P) && kinetic > 0.5f * (U.E + shift);
     float p;
-    if (interface_cell<M>(R.alpha)) {
+    if (interface_cell<M>(R.alpha) || fast_liquid) {
         // FP64 recovery: the energy minus kinetic energy difference, where the
         // cancellation sits, in FP64 (float products are exact in FP64); 1/rho
         // from the FP32 reciprocal and one Newton step (relative error ~1e-14)
@@ -365,19 +386,23 @@ __device__ __forceinline__ void cons_to_prim(const EP& P, const St<D, M>& U, Pr<
     R.p = fmaxs(p, 1e-10f);
 }
 
-// prim_to_cons (primitives.hpp:166-212) of a primitive whose fractions are
-// non-negative and sum to one (reconstructed faces after enforce); alpha
-// receives the floored fractions the state was built from
+// prim_to_cons (primitives.hpp:166-212) of a reconstructed face primitive:
+// its values lie between those of cons_to_prim outputs (MinBee faces), so
+// densities are >= 1e-12 and the pressure >= 1e-10, and for two materials
+// the fractions are >= 1e-12 (three or more materials: non-negative and
+// summing to one after enforce); alpha receives the floored fractions the
+// state was built from.  The reference's floors at these values are
+// therefore left out where they cannot act.
 template<int D, int M, typename EP>
 __device__ __forceinline__ void prim_to_cons(const EP& P, const Pr<D, M>& Q, St<D, M>& U, float* alpha)
 {
 #pragma unroll
     for (int k = 0; k < M; ++k) alpha[k] = Q.alpha[k];
-    floor_fractions<M>(alpha);
+    if constexpr (M > 2) floor_fractions<M>(alpha);
     float rho = -0.f, coef[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-        const float rk = fmaxs(Q.rho[k], 1e-12f);
+        const float rk = Q.rho[k];
         U.m[k] = alpha[k] * rk;
         coef[k] = ecoef(P, k, rk);
         rho += U.m[k];
@@ -396,7 +421,7 @@ __device__ __forceinline__ void prim_to_cons(const EP& P, const Pr<D, M>& Q, St<
         decode_alpha<M>(U.a, ad);
         shift = bshift<M>(P, alpha, ad);
     }
-    U.E = energy_shifted<M>(P, fmaxs(Q.p, 1e-12f), alpha, coef) + shift + 0.5f * rho * v2;
+    U.E = energy_shifted<M>(P, Q.p, alpha, coef) + shift + 0.5f * rho * v2;
 }
 
 // repair_state (primitives.hpp:326-457) on a state whose partial masses are
@@ -560,28 +585,27 @@ __device__ __forceinline__ float compressibility(const EP& P, float p, const flo
 {
     float beta = -0.f;
 #pragma unroll
-    for (int k = 0; k < M; ++k) {
-        const float a = fmaxs(alpha[k], 0.f);
-        beta += a * rcp(fmaxs(impedance(P, k, p, ecoef(P, k, rho_k[k])), 1e-12f));
-    }
+    for (int k = 0; k < M; ++k) beta += alpha[k] * rcp(fmaxs(impedance(P, k, p, ecoef(P, k, rho_k[k])), 1e-12f));
     return beta;
 }
 
 // Wood sound speed (eos.hpp:195-221) and the allaire / equal_velocity
-// variants (eos.hpp:223-275), closed form for ideal and stiffened gases
+// variants (eos.hpp:223-275), closed form for ideal and stiffened gases.
+// Its inputs are cons_to_prim outputs or reconstructed faces: p >= 1e-10
+// and fractions >= 1e-12 (non-negative for three or more materials), so the
+// reference's floors on them cannot act; the density floor stays.
 template<int M, typename EP>
 __device__ __forceinline__ float sound_speed(const EP& P, float rho, float p, const float* alpha, const float* rho_k)
 {
     rho = fmaxs(rho, 1e-10f);
-    p = fmaxs(p, 1e-10f);
     if (law_lambda(P) == LAMBDA_KAPILA) {
         if constexpr (M == 2) {
             // c^2 = 1/(rho (a0/I0 + a1/I1)) = Y/(rho X), X = a0 I1 + a1 I0, Y = I0 I1,
-            // so c = Y rsq(rho X Y): one transcendental
+            // so c = Y rsq(rho X Y): one transcendental (X >= 2e-24)
             const float I0 = fmaxs(impedance(P, 0, p, ecoef(P, 0, rho_k[0])), 1e-12f);
             const float I1 = fmaxs(impedance(P, 1, p, ecoef(P, 1, rho_k[1])), 1e-12f);
-            const float a0 = fmaxs(alpha[0], 0.f), a1 = fmaxs(alpha[1], 0.f);
-            const float X = fmaxs(a0 * I1 + a1 * I0, 1e-30f);
+            const float a0 = alpha[0], a1 = alpha[1];
+            const float X = a0 * I1 + a1 * I0;
             const float Y = I0 * I1;
             return Y * frsq(rho * X * Y);
         } else {
@@ -594,7 +618,7 @@ __device__ __forceinline__ float sound_speed(const EP& P, float rho, float p, co
     float cs[M], xi[M], xi_mix = -0.f, inv_sum = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-        const float a = fmaxs(alpha[k], 0.f);
+        const float a = alpha[k];
         xi[k] = ecoef(P, k, rho_k[k]);
         cs[k] = fsqrt(impedance(P, k, p, xi[k]) * rcp(fmaxs(rho_k[k], 1e-10f)));
         xi_mix += a * xi[k];
@@ -603,7 +627,7 @@ __device__ __forceinline__ float sound_speed(const EP& P, float rho, float p, co
     float rc2 = -0.f;
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-        const float a = fmaxs(alpha[k], 0.f);
+        const float a = alpha[k];
         const float lam = (law_lambda(P) == LAMBDA_EQUAL_VELOCITY) ? rcp(fmaxs(cs[k], 1e-14f) * inv_sum) : 1.f;
         rc2 += lam * a * xi[k] * cs[k] * cs[k];
     }
@@ -680,18 +704,17 @@ __device__ __forceinline__ void alpha_rhs(const EP& P, const Pr<D, M>& Q, float*
     }
 }
 
-// physical flux (flux.hpp:11-43) with the true energy E = E_hat + B(alpha)
+// physical flux (flux.hpp:11-43) with the true energy E = E_hat + B(alpha).
+// The face states here are built by prim_to_cons, whose momentum is rho u
+// of the primitive, so rho u_d u_n is taken as mom_d u_n.
 template<int D, int M, int DIR, typename EP>
 __device__ __forceinline__ void flux_normal(const EP& P, const St<D, M>& U, const Pr<D, M>& Q, Fx<D, M>& F)
 {
-    float rho = -0.f;
-#pragma unroll
-    for (int k = 0; k < M; ++k) rho += U.m[k];
     const float un = Q.vel[DIR];
 #pragma unroll
     for (int k = 0; k < M; ++k) F.m[k] = U.m[k] * un;
 #pragma unroll
-    for (int d = 0; d < D; ++d) F.mom[d] = rho * Q.vel[d] * un + (d == DIR ? Q.p : 0.f);
+    for (int d = 0; d < D; ++d) F.mom[d] = U.mom[d] * un + (d == DIR ? Q.p : 0.f);
     float B = -0.f;
     if (law_stiff(P)) {
         float ad[M];
@@ -705,12 +728,11 @@ __device__ __forceinline__ void flux_normal(const EP& P, const St<D, M>& U, cons
 template<int D, int M, int DIR, typename EP>
 __device__ __forceinline__ void hancock(const EP& P, const St<D, M>& Uc, const Pr<D, M>& Pc, const St<D, M>& ul,
                                         const St<D, M>& ur, const Pr<D, M>& lf, const Pr<D, M>& rf, const float* rhs,
-                                        float dt, float dx, St<D, M>& out)
+                                        float dt, float inv_dx, St<D, M>& out)
 {
     Fx<D, M> FL, FR;
     flux_normal<D, M, DIR>(P, ul, lf, FL);
     flux_normal<D, M, DIR>(P, ur, rf, FR);
-    const float inv_dx = rcp(dx);
     const float factor = 0.5f * dt * inv_dx;
     out = Uc;
 #pragma unroll
@@ -748,14 +770,22 @@ __device__ __forceinline__ int hllc(const EP& P, const St<D, M>& UL, const St<D,
         rhoL += UL.m[k];
         rhoR += UR.m[k];
     }
-    if (!ffinite(rhoL)) return ERR_HLLC_LEFT_DENSITY;
-    if (!ffinite(rhoR)) return ERR_HLLC_RIGHT_DENSITY;
-    if (!ffinite(PL.p)) return ERR_HLLC_LEFT_PRESSURE;
-    if (!ffinite(PR.p)) return ERR_HLLC_RIGHT_PRESSURE;
+    // require_positive (hllc.hpp:88-91) in the reference's order.  The four
+    // values are non-negative or not finite, so their sum is finite exactly
+    // when all four are (barring an overflow of the sum, which the
+    // individual checks then clear).  Face pressures are >= 1e-10 (cons_to_prim
+    // outputs and faces between them), so the pressure clamp to 1e-10 cannot
+    // act; the density clamp stays (an end cell's FP32 masses can round to 0)
+    if (!ffinite((rhoL + rhoR) + (PL.p + PR.p))) {
+        if (!ffinite(rhoL)) return ERR_HLLC_LEFT_DENSITY;
+        if (!ffinite(rhoR)) return ERR_HLLC_RIGHT_DENSITY;
+        if (!ffinite(PL.p)) return ERR_HLLC_LEFT_PRESSURE;
+        if (!ffinite(PR.p)) return ERR_HLLC_RIGHT_PRESSURE;
+    }
     rhoL = rhoL <= 0.f ? 1e-10f : rhoL;
     rhoR = rhoR <= 0.f ? 1e-10f : rhoR;
-    const float pL = PL.p <= 0.f ? 1e-10f : PL.p;
-    const float pR = PR.p <= 0.f ? 1e-10f : PR.p;
+    const float pL = PL.p;
+    const float pR = PR.p;
     const float unL = PL.vel[DIR], unR = PR.vel[DIR];
     const float cL = sound_speed<M>(P, rhoL, pL, PL.alpha, PL.rho);
     const float cR = sound_speed<M>(P, rhoR, pR, PR.alpha, PR.rho);
@@ -1084,8 +1114,10 @@ struct Fp32MixedT {
     }
 
     // face state and its primitive: when repair has nothing to do the
-    // primitive of prim_to_cons(Q) is Q with the same floors, so the
-    // pressure round trip is skipped
+    // primitive of prim_to_cons(Q) is Q with the same floors, which the face
+    // already satisfies (densities >= 1e-12, p >= 1e-10; the fractions are
+    // the ones the state was built from), so the pressure round trip is
+    // skipped
     template<int D, int M>
     static __device__ __forceinline__ void face_state(const Eos& P, const Pr<D, M>& Q, St<D, M>& U, Pr<D, M>& QU)
     {
@@ -1099,19 +1131,26 @@ struct Fp32MixedT {
 #pragma unroll
         for (int k = 0; k < M; ++k) {
             QU.alpha[k] = alpha[k];
-            QU.rho[k] = f32::fmaxs(Q.rho[k], 1e-12f);
+            QU.rho[k] = Q.rho[k];
         }
 #pragma unroll
         for (int d = 0; d < D; ++d) QU.vel[d] = Q.vel[d];
-        QU.p = f32::fmaxs(Q.p, 1e-10f);
+        QU.p = Q.p;
     }
+    // a sweep's step sizes: dt and the host's 1/dx (uniform values that stay
+    // in scalar registers); the FP32 factors are formed where they are used
+    struct Factors {
+        double dt, inv_dx;
+    };
+    static __device__ __forceinline__ Factors factors(double dt, double /*dx*/, double inv_dx) { return {dt, inv_dx}; }
+
     template<int D, int M, int DIR>
     static __device__ __forceinline__ void hancock(const Eos& P, const St<D, M>& Uc, const Pr<D, M>& Pc,
                                                    const St<D, M>& ul, const St<D, M>& ur, const Pr<D, M>& lf,
-                                                   const Pr<D, M>& rf, const float* rhs, double dt, double dx,
+                                                   const Pr<D, M>& rf, const float* rhs, const Factors& f,
                                                    St<D, M>& out)
     {
-        f32::hancock<D, M, DIR>(P, Uc, Pc, ul, ur, lf, rf, rhs, (float)dt, (float)dx, out);
+        f32::hancock<D, M, DIR>(P, Uc, Pc, ul, ur, lf, rf, rhs, (float)f.dt, (float)f.inv_dx, out);
     }
     template<int D, int M, int DIR>
     static __device__ __forceinline__ int hllc(const Eos& P, const St<D, M>& UL, const St<D, M>& UR,
@@ -1124,10 +1163,10 @@ struct Fp32MixedT {
     static __device__ __forceinline__ void update(const Eos& P, const Cell<D, M>& Uin, const Pr<D, M>& Pn,
                                                   const float* rhs, const Fx<D, M>& Fl, const Fx<D, M>& Fr,
                                                   float fvl, float fvr,
-                                                  const float* afl, const float* afr, double dt, double dx,
+                                                  const float* afl, const float* afr, const Factors& fac,
                                                   Cell<D, M>& r)
     {
-        const double lambda = dt / dx;
+        const double lambda = fac.dt * fac.inv_dx;
         const float lf = (float)lambda;
         r = Uin;
 #pragma unroll
@@ -1137,8 +1176,8 @@ struct Fp32MixedT {
         r.E -= lf * (Fr.E - Fl.E);
         f32::repair<D, M>(P, r);
         float an[M];
-        const float div_u = (fvr - fvl) * (float)(1.0 / dx);
-        const float dtf = (float)dt;
+        const float div_u = (fvr - fvl) * (float)fac.inv_dx;
+        const float dtf = (float)fac.dt;
 #pragma unroll
         for (int k = 0; k < M; ++k) an[k] = Pn.alpha[k] - lf * (afr[k] - afl[k]) + dtf * rhs[k] * div_u;
         f32::sanitise<M>(an, 0.f);
@@ -1232,3 +1271,5 @@ using Fp32Mixed = Fp32MixedT<EosMixed>;
 using Fp32Ideal = Fp32MixedT<EosMixedIdeal>;
 
 }  // namespace dimhip
+
+#pragma clang fp contract(off)

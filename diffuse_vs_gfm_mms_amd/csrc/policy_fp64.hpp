@@ -210,13 +210,20 @@ struct Fp64T {
         repair_state<D, M>(P, U);
         dimhip::cons_to_prim<D, M>(P, U, QU);
     }
+    // a sweep's step sizes: dt and dx, which the reference's expressions
+    // combine themselves (dt/dx, 0.5*dt/dx) for their bits
+    struct Factors {
+        double dt, dx;
+    };
+    static __device__ __forceinline__ Factors factors(double dt, double dx, double /*inv_dx*/) { return {dt, dx}; }
+
     template<int D, int M, int DIR>
     static __device__ __forceinline__ void hancock(const Eos& P, const St<D, M>& Uc, const Pr<D, M>& Pc,
                                                    const St<D, M>& ul, const St<D, M>& ur, const Pr<D, M>& lf,
-                                                   const Pr<D, M>& rf, const double* rhs, double dt, double dx,
+                                                   const Pr<D, M>& rf, const double* rhs, const Factors& f,
                                                    St<D, M>& out)
     {
-        hancock_predict<D, M, DIR>(P, Uc, Pc, ul, ur, lf, rf, rhs, dt, dx, out);
+        hancock_predict<D, M, DIR>(P, Uc, Pc, ul, ur, lf, rf, rhs, f.dt, f.dx, out);
     }
     template<int D, int M, int DIR>
     static __device__ __forceinline__ int hllc(const Eos& P, const St<D, M>& UL, const St<D, M>& UR,
@@ -230,9 +237,10 @@ struct Fp64T {
     static __device__ __forceinline__ void update(const Eos& P, const Cell<D, M>& Uin, const Pr<D, M>& Pn,
                                                   const double* rhs, const Fx<D, M>& Fl, const Fx<D, M>& Fr,
                                                   double fvl, double fvr,
-                                                  const double* afl, const double* afr, double dt, double dx,
+                                                  const double* afl, const double* afr, const Factors& f,
                                                   Cell<D, M>& r)
     {
+        const double dt = f.dt, dx = f.dx;
         const double lambda = dt / dx;
         r = Uin;
 #pragma unroll

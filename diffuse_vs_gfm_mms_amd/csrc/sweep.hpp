@@ -263,8 +263,7 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     __shared__ R sh[(NREC + NPARK) * S];
 
     __syncthreads();  // a block may run several tiles (batched launches): LDS reuse
-    const double dt = ctl->dt;
-    const double dxs = g.dx[DIR];
+    const typename Pol::Factors fac = Pol::factors(ctl->dt, g.dx[DIR], cflargs.inv_dx[DIR]);
 
     int pos, lane;
     if (DIR == 0) {
@@ -395,7 +394,7 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
             Pol::template face_cons<D, M>(P, rf, ur);
             R rhs_l[M];
             Pol::template alpha_rhs_line<D, M>(P, Pl, rhs_n, rhs_l);
-            Pol::template hancock<D, M, DIR>(P, U, Pl, ul, ur, lf, rf, rhs_l, dt, dxs, Uh);
+            Pol::template hancock<D, M, DIR>(P, U, Pl, ul, ur, lf, rf, rhs_l, fac, Uh);
             Pol::template cons_to_prim<D, M>(P, Uh, Ph);
             Pol::template enforce_c2p<D, M>(Ph);
         }
@@ -525,7 +524,7 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
 #pragma unroll
                 for (int k = 0; k < M; ++k) afr[k] = Pn.alpha[k] * fvr;
             }
-            Pol::template update<D, M>(P, Uin, Pn, rhs_n, Fl, Fr, fvl, fvr, afl, afr, dt, dxs, r);
+            Pol::template update<D, M>(P, Uin, Pn, rhs_n, Fl, Fr, fvl, fvr, afl, afr, fac, r);
         }
         if (MODE == 1) {
             Cell acc;
