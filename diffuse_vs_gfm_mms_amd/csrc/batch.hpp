@@ -28,16 +28,22 @@ struct Work {
 };
 
 // schedule: 0 = cyclic (block k takes entries k, k+G, ...), 1 = contiguous
-// chunks (block k takes [k*chunk, (k+1)*chunk))
+// chunks (block k takes [k*chunk, (k+1)*chunk)).  k is the block's logical
+// index in XCD order (xcd_block): the blocks that run on one XCD take
+// consecutive k, so the tiles one XCD works on at a time are neighbours in
+// the work table, whose halo rows and shared cache lines then stay in that
+// XCD's L2.
 __device__ __forceinline__ void work_range(int nwork, int schedule, int& beg, int& end, int& stride)
 {
+    int k, unused_y, unused_z;
+    xcd_block(k, unused_y, unused_z);
     if (schedule == 0) {
-        beg = blockIdx.x;
+        beg = k;
         end = nwork;
         stride = gridDim.x;
     } else {
         const int chunk = (nwork + gridDim.x - 1) / gridDim.x;
-        beg = blockIdx.x * chunk;
+        beg = k * chunk;
         end = min(nwork, beg + chunk);
         stride = 1;
     }

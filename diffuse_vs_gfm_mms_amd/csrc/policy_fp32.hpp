@@ -885,8 +885,16 @@ struct Fp32MixedT {
     static constexpr bool kReloadInput = false;
     // the update's inputs from the first phase (P^n's volume fractions and
     // the alpha source coefficients) wait in LDS while the reconstruction and
-    // flux phases run: no spill at the 8-wave cap, 3 % faster
+    // flux phases run, where the LDS budget has room for them (3D and
+    // three-material tiles)
     static constexpr bool kParkAlpha = true;
+    // 1D/2D two-material x tiles exchange through two alternating LDS
+    // regions with compact face records (18 floats per slot in 2D): x sweep
+    // 1 % faster.  The y tiles keep one region with the parked terms: with
+    // two regions they ran 5 % slower unparked (the parked values spill) and
+    // 10 % slower parked (22 floats per slot: three tiles per CU).
+    template<int D, int M, int DIR>
+    static constexpr bool ping_pong() { return D <= 2 && M <= 2 && DIR == 0; }
     // sweep tiles: x 128 positions x 2 lines; y/z 8 lines x 64 positions
     static constexpr int kSweepL0 = 128, kSweepW0 = 2, kSweepL1 = 64, kSweepW1 = 8;
     // CFL reduction: each cell contributes its wave rate max_d (|u_d|+c)/(cfl dx_d),
@@ -1111,6 +1119,22 @@ struct Fp32MixedT {
         float alpha[M];
         f32::prim_to_cons<D, M>(P, Q, U, alpha);
         if (needs_repair<D, M>(U, alpha)) f32::repair<D, M>(P, U);
+    }
+
+    // the face state of a compact face record (its primitive Q, momentum and
+    // energy): the partial masses and the fraction code are those
+    // prim_to_cons builds from Q (alpha_k rho_k, the code of Q's fractions),
+    // exactly so unless the face went through repair or is an end cell,
+    // where they agree to rounding
+    template<int D, int M>
+    static __device__ __forceinline__ void face_from_record(const Pr<D, M>& Q, const float* mom, float E, St<D, M>& U)
+    {
+#pragma unroll
+        for (int k = 0; k < M; ++k) U.m[k] = Q.alpha[k] * Q.rho[k];
+#pragma unroll
+        for (int d = 0; d < D; ++d) U.mom[d] = mom[d];
+        U.E = E;
+        f32::encode_alpha<M>(Q.alpha, U.a);
     }
 
     // face state and its primitive: when repair has nothing to do the

@@ -39,6 +39,10 @@ struct Fp64T {
     // the alpha source coefficients) wait in LDS, not registers, while the
     // reconstruction and flux phases run (fewer spills at the 4-wave cap)
     static constexpr bool kParkAlpha = true;
+    // one LDS region for every exchange of the sweep tile: the faces' full
+    // FP64 states travel through LDS (two regions would halve the occupancy)
+    template<int D, int M, int DIR>
+    static constexpr bool ping_pong() { return false; }
     // sweep tiles: x 128 positions x 2 lines; y/z 4 lines x 128 positions
     // (halo 6 of 128; 8 lines x 64 positions was 1.6 % slower)
     static constexpr int kSweepL0 = 128, kSweepW0 = 2, kSweepL1 = 128, kSweepW1 = 4;

@@ -117,40 +117,47 @@ __device__ __forceinline__ unsigned long long wave_reduce_u64(unsigned long long
     return ((unsigned long long)hi << 32) | lo;
 }
 
-// Max over a full wave of non-negative doubles (wave rates): DPP moves of the
-// two halves and an FP64 max per step, lane 63 ends with the result.  A NaN
-// rate never wins (IEEE maxNum), as a NaN candidate never wins the
-// reference's running std::min on dt.
-__device__ __forceinline__ double wave_max_f64(double v)
+// Max over a full wave of 32-bit unsigned values: a DPP move and an
+// integer max per step (quad swaps, half-row and row mirrors, row broadcasts
+// 15 and 31); lane 63 ends with the result, which is returned to every lane.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v)
 {
-    const unsigned long long idle = 0ull;  // +0.0
-    const auto step = [&](unsigned long long o) { v = __builtin_fmax(v, bits_to_double(o)); };
-    step(dpp_u64<0xB1, 0xF>(double_to_bits(v), idle));   // quad_perm [1,0,3,2]
-    step(dpp_u64<0x4E, 0xF>(double_to_bits(v), idle));   // quad_perm [2,3,0,1]
-    step(dpp_u64<0x141, 0xF>(double_to_bits(v), idle));  // row_half_mirror
-    step(dpp_u64<0x140, 0xF>(double_to_bits(v), idle));  // row_mirror
-    step(dpp_u64<0x142, 0xA>(double_to_bits(v), idle));  // row_bcast:15 into rows 1, 3
-    step(dpp_u64<0x143, 0xC>(double_to_bits(v), idle));  // row_bcast:31 into rows 2, 3
-    const unsigned long long b = double_to_bits(v);
-    const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)b, 63);
-    const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
-    return bits_to_double(((unsigned long long)hi << 32) | lo);
+    const auto step = [&](unsigned o) { v = v > o ? v : o; };
+    step((unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    step((unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    step((unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    step((unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false));  // row_mirror
+    step((unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false));  // row_bcast:15 into rows 1, 3
+    step((unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false));  // row_bcast:31 into rows 2, 3
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
 }
 
-// the wave's combined CFL value as a bit pattern: rates by FP64 max,
-// candidates by the bit-ordered min
+// Max over a full wave of non-negative, non-NaN doubles (the wave rates of
+// the FP32-mixed CFL: 0 or positive, cfl_value never yields NaN), as their
+// bit patterns, which order like unsigned integers: the wave's largest high
+// word first, then the largest low word among the lanes that hold it.  The
+// maximum is exact, so this equals an FP64 max reduction.
+__device__ __forceinline__ unsigned long long wave_max_bits_nonneg(double v)
+{
+    const unsigned long long b = double_to_bits(v);
+    const unsigned hi = wave_max_u32((unsigned)(b >> 32));
+    const unsigned lo = wave_max_u32(((unsigned)(b >> 32) == hi) ? (unsigned)b : 0u);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// the wave's combined CFL value as a bit pattern: rates by max, candidates
+// by the bit-ordered min
 template<typename Pol>
 __device__ __forceinline__ unsigned long long wave_cfl(double v)
 {
-    if constexpr (Pol::kCflRate) return double_to_bits(wave_max_f64(v));
+    if constexpr (Pol::kCflRate) return wave_max_bits_nonneg(v);
     else return wave_reduce_u64<false>(double_to_bits(v));
 }
 
 template<typename Pol>
 __device__ __forceinline__ unsigned long long cfl_combine_bits(unsigned long long a, unsigned long long b)
 {
-    if constexpr (Pol::kCflRate) return double_to_bits(__builtin_fmax(bits_to_double(a), bits_to_double(b)));
-    else return u64_combine<false>(a, b);
+    return u64_combine<Pol::kCflRate>(a, b);
 }
 
 // CFL reduction of one block into its dt slots (bit-ordered atomicMin of a
@@ -253,16 +260,32 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     using Cell = typename Pol::template Cell<D, M>;
     constexpr int S = L * W;
     constexpr int NV = 2 * M + D;
-    constexpr int NREC = 4 * M + 2 * D + 1;
+    constexpr int NPR = 2 * M + D + 1;  // primitive record: rho[M], alpha[M], vel[D], p
     constexpr int T = L - 6;
+    // LDS records, SoA rows of S slots.  Pol::ping_pong: consecutive
+    // exchanges alternate between two regions, A (P_line, then the face
+    // records) and B (the half-step primitives, then the fluxes), so a phase
+    // writes its output as soon as it is computed -- the region it overwrites
+    // was last read before the previous barrier -- and each exchange takes one
+    // barrier.  The face record is then the face primitive with the face
+    // state's momentum and energy (its partial masses and fraction code are
+    // rebuilt from the primitive).  Otherwise one region holds every
+    // exchange, the full face state and primitive, with a barrier on each
+    // side of every write.
+    constexpr bool PP = Pol::template ping_pong<D, M, DIR>();
+    constexpr int NFACE = PP ? NPR + D + 1 : NV + NPR;
+    constexpr int NFLX = 2 * M + D + 2;  // F.m[M], F.mom[D], F.E, face velocity, alpha flux[M]
+    constexpr int OFF_B = PP ? (NPR > NFACE ? NPR : NFACE) : 0;
+    constexpr int NREC = PP ? OFF_B + (NPR > NFLX ? NPR : NFLX) : NV + NPR;
     // Pol::kParkAlpha: P^n's fractions and rhs_n wait in NPARK extra LDS rows
     // of the thread's own slot from phase 1 to phase 5 (for nmat <= 2, whose
-    // tiles still fit the occupancy the register cap allows)
-    constexpr bool PARK = Pol::kParkAlpha && M <= 2;
+    // tiles still fit the occupancy the register cap allows; not with the
+    // two regions, whose records already fill that budget)
+    constexpr bool PARK = Pol::kParkAlpha && M <= 2 && !PP;
     constexpr int NPARK = PARK ? 2 * M : 0;
     __shared__ R sh[(NREC + NPARK) * S];
 
-    __syncthreads();  // a block may run several tiles (batched launches): LDS reuse
+    if (PERSISTENT) __syncthreads();  // the block's previous tile may still read the LDS
     const typename Pol::Factors fac = Pol::factors(ctl->dt, g.dx[DIR], cflargs.inv_dx[DIR]);
 
     int pos, lane;
@@ -398,9 +421,12 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
             Pol::template cons_to_prim<D, M>(P, Uh, Ph);
             Pol::template enforce_c2p<D, M>(Ph);
         }
+        if (PP) put_prim<R, D, M>(sh, S, slot, OFF_B, Ph);
     }
-    __syncthreads();
-    if (ph2) put_prim<R, D, M>(sh, S, slot, 0, Ph);
+    if (!PP) {
+        __syncthreads();
+        if (ph2) put_prim<R, D, M>(sh, S, slot, 0, Ph);
+    }
     __syncthreads();
 
     // phase 3
@@ -425,17 +451,25 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
             PUr = PUl;
         } else {
             Pr PL, PR, lf, rf;
-            get_prim<R, D, M>(sh, S, slot_l, 0, PL);
-            get_prim<R, D, M>(sh, S, slot_r, 0, PR);
+            get_prim<R, D, M>(sh, S, slot_l, OFF_B, PL);
+            get_prim<R, D, M>(sh, S, slot_r, OFF_B, PR);
             Pol::template reconstruct<D, M>(PL, Ph, PR, lf, rf);
             Pol::template face_state<D, M>(P, lf, Ulh, PUl);
             Pol::template face_state<D, M>(P, rf, Urh, PUr);
         }
+        if constexpr (PP) {
+            put_prim<R, D, M>(sh, S, slot, 0, PUl);
+#pragma unroll
+            for (int d = 0; d < D; ++d) sh[(NPR + d) * S + slot] = Ulh.mom[d];
+            sh[(NPR + D) * S + slot] = Ulh.E;
+        }
     }
-    __syncthreads();
-    if (ph3) {
-        put_state<R, D, M>(sh, S, slot, 0, Ulh);
-        put_prim<R, D, M>(sh, S, slot, NV, PUl);
+    if (!PP) {
+        __syncthreads();
+        if (ph3) {
+            put_state<R, D, M>(sh, S, slot, 0, Ulh);
+            put_prim<R, D, M>(sh, S, slot, NV, PUl);
+        }
     }
     __syncthreads();
 
@@ -447,8 +481,16 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     if (ph4) {
         St UR;
         Pr PR;
-        get_state<R, D, M>(sh, S, slot_r, 0, UR);
-        get_prim<R, D, M>(sh, S, slot_r, NV, PR);
+        if constexpr (PP) {
+            get_prim<R, D, M>(sh, S, slot_r, 0, PR);
+            R mom[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) mom[d] = sh[(NPR + d) * S + slot_r];
+            Pol::template face_from_record<D, M>(PR, mom, sh[(NPR + D) * S + slot_r], UR);
+        } else {
+            get_state<R, D, M>(sh, S, slot_r, 0, UR);
+            get_prim<R, D, M>(sh, S, slot_r, NV, PR);
+        }
         if (const int e = Pol::template hllc<D, M, DIR>(P, Urh, UR, PUr, PR, F, fv)) {
             // the reference's line index: the other axes ascending, last fastest
             long long line = 0;
@@ -462,9 +504,8 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
 #pragma unroll
         for (int k = 0; k < M; ++k) af[k] = up[k] * fv;
     }
-    __syncthreads();
-    if (ph4) {
-        int f = 0;
+    const auto put_flux = [&]() {
+        int f = OFF_B;
 #pragma unroll
         for (int k = 0; k < M; ++k) sh[(f++) * S + slot] = F.m[k];
 #pragma unroll
@@ -473,6 +514,12 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
         sh[(f++) * S + slot] = fv;
 #pragma unroll
         for (int k = 0; k < M; ++k) sh[(f++) * S + slot] = af[k];
+    };
+    if (PP) {
+        if (ph4) put_flux();
+    } else {
+        __syncthreads();
+        if (ph4) put_flux();
     }
     __syncthreads();
 
@@ -502,7 +549,7 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
                 for (int k = 0; k < M; ++k) afr[k] = af[k];
             }
             if (i >= 1) {
-                int f = 0;
+                int f = OFF_B;
 #pragma unroll
                 for (int k = 0; k < M; ++k) Fl.m[k] = sh[(f++) * S + slot_l];
 #pragma unroll
