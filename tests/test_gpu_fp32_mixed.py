@@ -118,11 +118,13 @@ def test_water_air_tube_fp32_mixed(oracle):
     """Stiffened-gas water/air tube (1000 cells, 300 steps).  The FP32-mixed
     form is accurate for gas-gas mixtures; in nearly pure water expanding at
     high speed (p ~ 3e4 Pa << gamma p_inf = 2.6e9 Pa, u ~ 480 m/s) the shifted
-    internal energy and the kinetic energy cancel to 1e-3 and FP32 energy
-    storage leaves ~1e-3 relative pressure error there; the interface then
-    evolves measurably differently (FP64 itself moves by 4e-5 under 1e-7 input
-    noise).  Measured: rho 2.8e-3, p 2.8e-3, alpha 3.3e-3 normalised L1; the
-    test bounds them at 1e-2 and the FP64 form is the one for stiff liquids."""
+    internal energy is mostly kinetic (1 part in 1e4 is internal), so its
+    FP32 storage leaves ~1e-4 to 1e-3 relative pressure error there, even with
+    the kinetic energy subtracted in FP64 (which the form does in such fast
+    liquid cells as in interface cells); the interface then evolves
+    measurably differently (FP64 itself moves by 4e-5 under 1e-7 input
+    noise).  Measured: rho 5.0e-4, p 3.9e-4, alpha 6.4e-4 normalised L1; the
+    test bounds them at 2e-3 and the FP64 form is the one for stiff liquids."""
     c = W.water_air_tube_1d(1000)
     U64, _, _, _, _ = run(c, "fp64", 300)
     U32, _, _, _, _ = run(c, "fp32_mixed", 300)
@@ -130,7 +132,7 @@ def test_water_air_tube_fp32_mixed(oracle):
     r32, p32, a32 = fields(oracle, c, U32)
     errs = (l1(r32, r64), l1(p32, p64), l1(a32, a64))
     print(f"water-air L1: rho {errs[0]:.3e} p {errs[1]:.3e} alpha {errs[2]:.3e}")
-    assert max(errs) <= 1e-2, errs
+    assert max(errs) <= 2e-3, errs
 
 
 def test_fp32_mixed_3d_and_unsplit_run():
