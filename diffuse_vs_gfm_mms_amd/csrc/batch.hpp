@@ -66,6 +66,20 @@ batch_sweep_kernel(const BatchInst<Pol>* __restrict__ insts, const Work* __restr
     }
 }
 
+// schedule 2: one work entry per block (no persistent loop), entries in
+// XCD order as in the single-grid sweeps
+template<typename Pol, int DIR, int L, int W, bool FUSE>
+__global__ void __launch_bounds__(L* W) __attribute__((amdgpu_waves_per_eu(Pol::template min_waves_per_eu<2, 2>())))
+batch_tile_kernel(const BatchInst<Pol>* __restrict__ insts, const Work* __restrict__ work, int cur)
+{
+    int w, unused_y, unused_z;
+    xcd_block(w, unused_y, unused_z);
+    const Work it = work[w];
+    const BatchInst<Pol>& I = insts[it.inst];
+    sweep_tile<Pol, 2, 2, DIR, L, W, 0, FUSE, false>(I.q[cur], I.q[cur ^ 1], I.g, I.P, I.ctl, I.cfl, nullptr, nullptr,
+                                                      it.a, it.b, 0);
+}
+
 template<typename Pol>
 __global__ void __launch_bounds__(256)
 batch_bc_kernel(const BatchInst<Pol>* __restrict__ insts, const Work* __restrict__ work, int nwork, int cur,

@@ -1266,6 +1266,15 @@ template<int DIR, int L, int W, bool FUSE>
 void launch_bsweep(dimhip_batch* b, int k)
 {
     const dim3 blk = DIR == 0 ? dim3(L, W, 1) : dim3(W, L, 1);
+    if (b->schedule == 2) {
+        if (b->ideal)
+            batch_tile_kernel<Fp32Ideal, DIR, L, W, FUSE><<<b->nwork[k], blk, 0, b->stream>>>(
+                insts_as<Fp32Ideal>(b), b->d_work[k], b->cur);
+        else
+            batch_tile_kernel<BPol, DIR, L, W, FUSE><<<b->nwork[k], blk, 0, b->stream>>>(b->d_insts, b->d_work[k],
+                                                                                          b->cur);
+        return;
+    }
     const int g = sweep_blocks(b, b->nwork[k]);
     if (b->ideal)
         batch_sweep_kernel<Fp32Ideal, DIR, L, W, FUSE><<<g, blk, 0, b->stream>>>(
@@ -1417,7 +1426,7 @@ int dimhip_batch_download(dimhip_batch* b, int i, double* packed, size_t ncells)
 int dimhip_batch_configure(dimhip_batch* b, int x_variant, int y_variant, int schedule, int blocks_per_cu)
 {
     if (!b) return bfail(b, "dimhip_batch_configure: null batch");
-    if (x_variant < 0 || x_variant > 1 || y_variant < 0 || y_variant > 2 || schedule < 0 || schedule > 1 ||
+    if (x_variant < 0 || x_variant > 1 || y_variant < 0 || y_variant > 2 || schedule < 0 || schedule > 2 ||
         blocks_per_cu < 1 || blocks_per_cu > 64)
         return bfail(b, "dimhip_batch_configure: variant out of range");
     BHIP_OK(b, hipStreamSynchronize(b->stream));
@@ -1522,7 +1531,8 @@ int dimhip_batch_sweep_times(dimhip_batch* b, double* mean_ms, long long* counts
     return 0;
 }
 
-// Times every (x tile, y tile, schedule, blocks per CU) choice over
+// Times every (x tile, y tile, schedule, blocks per CU) choice (schedule 2:
+// one launch configuration, its block count fixed by the work table) over
 // trial_steps steps from the current state, restores the state, keeps the
 // fastest choice and describes the trials as JSON in report.
 int dimhip_batch_autotune(dimhip_batch* b, double cfl, int trial_steps, char* report, size_t report_len)
@@ -1555,8 +1565,10 @@ int dimhip_batch_autotune(dimhip_batch* b, double cfl, int trial_steps, char* re
     bool first = true;
     for (int xv = 0; xv < 2; ++xv)
         for (int yv = 0; yv < 3; ++yv)
-            for (int sc = 0; sc < 2; ++sc)
+            for (int sc = 0; sc < 3; ++sc)
                 for (int bi = 0; bi < 3; ++bi) {
+                    // one tile per block: the block count is the work table's
+                    if (sc == 2 && bi > 0) continue;
                     if (dimhip_batch_configure(b, xv, yv, sc, bpcs[bi])) return 1;
                     for (int i = 0; i < n; ++i) times[i] = ctl_backup[i].time;
                     // one untimed step settles dt, then the timed trial

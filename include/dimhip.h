@@ -275,8 +275,10 @@ int dimhip_batch_download(dimhip_batch* batch, int instance, double* packed, siz
 
 /* Tile shape and schedule: x_variant 0 = 128 x 2, 1 = 256 x 1 (positions x
  * lines); y_variant 0 = 8 x 64, 1 = 16 x 32, 2 = 4 x 128 (lines x positions);
- * schedule 0 = cyclic, 1 = contiguous chunks of the work table per block;
- * blocks_per_cu = persistent blocks launched per compute unit. */
+ * schedule 0 = cyclic, 1 = contiguous chunks of the work table per
+ * persistent block, 2 = one work entry per block (no persistent loop);
+ * blocks_per_cu = persistent blocks launched per compute unit (schedules 0
+ * and 1; schedule 2 launches one block per work entry). */
 int dimhip_batch_configure(dimhip_batch* batch, int x_variant, int y_variant, int schedule, int blocks_per_cu);
 
 /* Times every configuration over trial_steps steps from the current state,

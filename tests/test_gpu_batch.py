@@ -20,7 +20,7 @@ def alone(case, nsteps):
         return s.download(), t, n
 
 
-@pytest.mark.parametrize("cfg", [(0, 0, 1, 4), (1, 1, 0, 2), (0, 2, 0, 8), (1, 0, 1, 1)])
+@pytest.mark.parametrize("cfg", [(0, 0, 1, 4), (1, 1, 0, 2), (0, 2, 0, 8), (1, 0, 1, 1), (0, 1, 2, 1), (1, 2, 2, 1)])
 def test_batch_equals_single_instances(cfg):
     cases = small_batch()
     refs = [alone(c, 7) for c in cases]
@@ -39,7 +39,8 @@ def test_batch_graph_and_autotune_keep_results():
     refs = [alone(c, 9) for c in cases]
     with dimhip.Batch(cases) as b:
         report = b.autotune(0.8, trial_steps=2)
-        assert "best" in report and len(report["trials"]) == 36
+        # 2 x tiles x 3 y tiles x (2 persistent schedules x 3 block counts + one-tile blocks)
+        assert "best" in report and len(report["trials"]) == 42
         times, steps = b.advance(9, 0.8, use_graph=True)
         for i in range(len(cases)):
             assert bits_equal(b.download(i), refs[i][0]), i
