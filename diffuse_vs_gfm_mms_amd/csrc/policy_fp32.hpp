@@ -424,6 +424,28 @@ __device__ __forceinline__ void prim_to_cons(const EP& P, const Pr<D, M>& Q, St<
     U.E = energy_shifted<M>(P, Q.p, alpha, coef) + shift + 0.5f * rho * v2;
 }
 
+// a fraction at or below the inactive threshold (1e-10) that carries more
+// than 1e-12 of mass (primitives.hpp:392-399)
+template<int M, typename MT>
+__device__ __forceinline__ bool inactive_mass(const float* alpha, const MT* m)
+{
+    bool r = false;
+#pragma unroll
+    for (int k = 0; k < M; ++k) r |= (alpha[k] <= 1e-10f && m[k] > MT(1e-12));
+    return r;
+}
+
+// sum_k alpha_k bmin_k: the energy floor's internal-energy part at the
+// floor pressure
+template<int M, typename EP>
+__device__ __forceinline__ float energy_floor(const EP& P, const float* alpha)
+{
+    float emin = -0.f;
+#pragma unroll
+    for (int k = 0; k < M; ++k) emin += alpha[k] * P.bmin[k];
+    return emin;
+}
+
 // repair_state (primitives.hpp:326-457) on a state whose partial masses are
 // of type MT (float for face/half states, double for stored cells).  The
 // common case (valid fraction code, no inactive mass) is the energy floor
@@ -517,10 +539,7 @@ __device__ __forceinline__ void repair(const EP& P, S& U)
         const float v = U.mom[d] * inv;
         v2 += v * v;
     }
-    float emin = -0.f;
-#pragma unroll
-    for (int k = 0; k < M; ++k) emin += alpha[k] * P.bmin[k];
-    U.E = fmaxs(U.E, 0.5f * rho * v2 + emin + fix);
+    U.E = fmaxs(U.E, 0.5f * rho * v2 + energy_floor<M>(P, alpha) + fix);
 }
 
 // muscl.hpp:26-40
@@ -1198,23 +1217,60 @@ struct Fp32MixedT {
 #pragma unroll
         for (int d = 0; d < D; ++d) r.mom[d] -= lf * (Fr.mom[d] - Fl.mom[d]);
         r.E -= lf * (Fr.E - Fl.E);
-        f32::repair<D, M>(P, r);
         float an[M];
         const float div_u = (fvr - fvl) * (float)fac.inv_dx;
         const float dtf = (float)fac.dt;
 #pragma unroll
         for (int k = 0; k < M; ++k) an[k] = Pn.alpha[k] - lf * (afr[k] - afl[k]) + dtf * rhs[k] * div_u;
         f32::sanitise<M>(an, 0.f);
-        if (law_stiff(P)) {
-            float ad_old[M], ad_new[M];
-            f32::decode_alpha<M>(r.a, ad_old);
-            f32::encode_alpha<M>(an, r.a);
-            f32::decode_alpha<M>(r.a, ad_new);
-            r.E += f32::bshift<M>(P, ad_old, ad_new);
+        const auto set_fractions = [&]() {
+            if (law_stiff(P)) {
+                float ad_old[M], ad_new[M];
+                f32::decode_alpha<M>(r.a, ad_old);
+                f32::encode_alpha<M>(an, r.a);
+                f32::decode_alpha<M>(r.a, ad_new);
+                r.E += f32::bshift<M>(P, ad_old, ad_new);
+            } else {
+                f32::encode_alpha<M>(an, r.a);
+            }
+        };
+        // The reference repairs the state before and after the fraction
+        // update.  When neither repair renormalises a fraction code or
+        // removes inactive mass (and the density is positive), both are the
+        // energy floor alone, with the same partial masses and momentum: the
+        // density and kinetic term are formed once and each floor applied
+        // with its own fractions -- the same operations as the two full
+        // repairs, so the same bits.  Otherwise both run in full.
+#pragma unroll
+        for (int k = 0; k < M; ++k) r.m[k] = (r.m[k] > 0.0) ? r.m[k] : 0.0;
+        float ad0[M];
+        f32::decode_alpha<M>(r.a, ad0);
+        double rt = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) rt += r.m[k];
+        if (f32::valid_code<M>(r.a) && !f32::inactive_mass<M>(ad0, r.m) && rt > 0.0) {
+            const float rho = (float)rt;
+            const float inv = f32::rcp(f32::fmaxs(rho, 1e-10f));
+            float v2 = -0.f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const float v = r.mom[d] * inv;
+                v2 += v * v;
+            }
+            const float h = 0.5f * rho;
+            r.E = f32::fmaxs(r.E, h * v2 + f32::energy_floor<M>(P, ad0));
+            set_fractions();
+            float ad1[M];
+            f32::decode_alpha<M>(r.a, ad1);
+            if (f32::valid_code<M>(r.a) && !f32::inactive_mass<M>(ad1, r.m))
+                r.E = f32::fmaxs(r.E, h * v2 + f32::energy_floor<M>(P, ad1));
+            else
+                f32::repair<D, M>(P, r);
         } else {
-            f32::encode_alpha<M>(an, r.a);
+            f32::repair<D, M>(P, r);
+            set_fractions();
+            f32::repair<D, M>(P, r);
         }
-        f32::repair<D, M>(P, r);
     }
 
     template<int D, int M>
