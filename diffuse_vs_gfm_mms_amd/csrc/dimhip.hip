@@ -509,7 +509,11 @@ int dimhip_create(const dimhip_desc* desc, dimhip_ctx** out)
     }
     c->eosm.alpha_source_floor = (float)desc->alpha_source_floor;
     c->eosmi = EosMixedIdeal(c->eosm);
-    c->fp32_ideal = !c->eosm.stiff && desc->lambda_model == DIMHIP_LAMBDA_KAPILA;
+    // the ideal-gas kernels also take min(1, 1/(gamma-1)) = 1 as a constant:
+    // every material has gamma <= 2
+    bool unit_coef_lo = true;
+    for (int k = 0; k < desc->nmat; ++k) unit_coef_lo = unit_coef_lo && c->eosm.coef_lo[k] == 1.0f;
+    c->fp32_ideal = !c->eosm.stiff && desc->lambda_model == DIMHIP_LAMBDA_KAPILA && unit_coef_lo;
     // ring cells per face (d, side), owned by the face of their lowest boundary axis
     long long acc = 0;
     for (int f = 0; f < 6; ++f) {

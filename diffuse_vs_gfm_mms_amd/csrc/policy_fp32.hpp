@@ -104,6 +104,13 @@ __device__ __forceinline__ float ecoef(const EP& P, int k, float rho_k)
 {
     return __builtin_amdgcn_fmed3f(rho_k * 1e10f, P.coef_lo[k], P.inv_gm1[k]);
 }
+// EosMixedIdeal contexts have gamma_k <= 2 for every material (the host
+// selects these kernels only then), so min(1, 1/(gamma_k - 1)) is 1: an
+// inline constant instead of a second parameter register
+__device__ __forceinline__ float ecoef(const EosMixedIdeal& P, int k, float rho_k)
+{
+    return __builtin_amdgcn_fmed3f(rho_k * 1e10f, 1.0f, P.inv_gm1[k]);
+}
 
 // rho_k c_k^2 of the reference's material_sound_speed (src/dim/eos.hpp:29-40):
 // the pressure recomputed from the energy built at the clamped density,
