@@ -914,13 +914,18 @@ struct Fp32MixedT {
     // flux phases run, where the LDS budget has room for them (3D and
     // three-material tiles)
     static constexpr bool kParkAlpha = true;
-    // 1D/2D two-material x tiles exchange through two alternating LDS
-    // regions with compact face records (18 floats per slot in 2D): x sweep
-    // 1 % faster.  The y tiles keep one region with the parked terms: with
-    // two regions they ran 5 % slower unparked (the parked values spill) and
-    // 10 % slower parked (22 floats per slot: three tiles per CU).
+    // 1D/2D two-material tiles exchange through two alternating LDS
+    // regions.  x tiles carry face records of primitive, momentum and energy
+    // and hold the update's fraction terms in registers (18 floats per slot:
+    // eight 256-thread tiles per CU); y tiles carry the face primitive alone
+    // and park the fraction terms (19 floats per slot: four 512-thread tiles
+    // per CU, where the other records spilled or cost a tile per CU).
     template<int D, int M, int DIR>
-    static constexpr bool ping_pong() { return D <= 2 && M <= 2 && DIR == 0; }
+    static constexpr bool ping_pong() { return D <= 2 && M <= 2; }
+    template<int D, int M, int DIR>
+    static constexpr bool prim_faces() { return DIR != 0; }
+    template<int D, int M, int DIR>
+    static constexpr bool park() { return kParkAlpha && M <= 2 && !(ping_pong<D, M, DIR>() && DIR == 0); }
     // sweep tiles: x 128 positions x 2 lines; y/z 8 lines x 64 positions
     static constexpr int kSweepL0 = 128, kSweepW0 = 2, kSweepL1 = 64, kSweepW1 = 8;
     // CFL reduction: each cell contributes its wave rate max_d (|u_d|+c)/(cfl dx_d),
@@ -1161,6 +1166,16 @@ struct Fp32MixedT {
         for (int d = 0; d < D; ++d) U.mom[d] = mom[d];
         U.E = E;
         f32::encode_alpha<M>(Q.alpha, U.a);
+    }
+
+    // the face state of a face primitive record: prim_to_cons of it, which
+    // is the face state itself unless that went through repair or is an end
+    // cell (then the two agree to rounding)
+    template<int D, int M>
+    static __device__ __forceinline__ void face_from_prim(const Eos& P, const Pr<D, M>& Q, St<D, M>& U)
+    {
+        float alpha[M];
+        f32::prim_to_cons<D, M>(P, Q, U, alpha);
     }
 
     // face state and its primitive: when repair has nothing to do the

@@ -43,6 +43,10 @@ struct Fp64T {
     // FP64 states travel through LDS (two regions would halve the occupancy)
     template<int D, int M, int DIR>
     static constexpr bool ping_pong() { return false; }
+    template<int D, int M, int DIR>
+    static constexpr bool prim_faces() { return false; }
+    template<int D, int M, int DIR>
+    static constexpr bool park() { return kParkAlpha && M <= 2; }
     // sweep tiles: x 128 positions x 2 lines; y/z 4 lines x 128 positions
     // (halo 6 of 128; 8 lines x 64 positions was 1.6 % slower)
     static constexpr int kSweepL0 = 128, kSweepW0 = 2, kSweepL1 = 128, kSweepW1 = 4;

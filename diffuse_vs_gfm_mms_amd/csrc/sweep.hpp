@@ -269,11 +269,13 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     // was last read before the previous barrier -- and each exchange takes one
     // barrier.  The face record is then the face primitive with the face
     // state's momentum and energy (its partial masses and fraction code are
-    // rebuilt from the primitive).  Otherwise one region holds every
-    // exchange, the full face state and primitive, with a barrier on each
-    // side of every write.
+    // rebuilt from the primitive), or with Pol::prim_faces the face primitive
+    // alone (the state rebuilt from it by prim_to_cons).  Otherwise one region
+    // holds every exchange, the full face state and primitive, with a barrier
+    // on each side of every write.
     constexpr bool PP = Pol::template ping_pong<D, M, DIR>();
-    constexpr int NFACE = PP ? NPR + D + 1 : NV + NPR;
+    constexpr bool PRIM_FACES = PP && Pol::template prim_faces<D, M, DIR>();
+    constexpr int NFACE = PP ? (PRIM_FACES ? NPR : NPR + D + 1) : NV + NPR;
     constexpr int NFLX = 2 * M + D + 2;  // F.m[M], F.mom[D], F.E, face velocity, alpha flux[M]
     constexpr int OFF_B = PP ? (NPR > NFACE ? NPR : NFACE) : 0;
     constexpr int NREC = PP ? OFF_B + (NPR > NFLX ? NPR : NFLX) : NV + NPR;
@@ -281,7 +283,7 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     // of the thread's own slot from phase 1 to phase 5 (for nmat <= 2, whose
     // tiles still fit the occupancy the register cap allows; not with the
     // two regions, whose records already fill that budget)
-    constexpr bool PARK = Pol::kParkAlpha && M <= 2 && !PP;
+    constexpr bool PARK = Pol::template park<D, M, DIR>();
     constexpr int NPARK = PARK ? 2 * M : 0;
     __shared__ R sh[(NREC + NPARK) * S];
 
@@ -459,9 +461,11 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
         }
         if constexpr (PP) {
             put_prim<R, D, M>(sh, S, slot, 0, PUl);
+            if constexpr (!PRIM_FACES) {
 #pragma unroll
-            for (int d = 0; d < D; ++d) sh[(NPR + d) * S + slot] = Ulh.mom[d];
-            sh[(NPR + D) * S + slot] = Ulh.E;
+                for (int d = 0; d < D; ++d) sh[(NPR + d) * S + slot] = Ulh.mom[d];
+                sh[(NPR + D) * S + slot] = Ulh.E;
+            }
         }
     }
     if (!PP) {
@@ -481,7 +485,10 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     if (ph4) {
         St UR;
         Pr PR;
-        if constexpr (PP) {
+        if constexpr (PRIM_FACES) {
+            get_prim<R, D, M>(sh, S, slot_r, 0, PR);
+            Pol::template face_from_prim<D, M>(P, PR, UR);
+        } else if constexpr (PP) {
             get_prim<R, D, M>(sh, S, slot_r, 0, PR);
             R mom[D];
 #pragma unroll
