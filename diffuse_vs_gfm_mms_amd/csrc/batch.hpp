@@ -51,7 +51,9 @@ __device__ __forceinline__ void work_range(int nwork, int schedule, int& beg, in
 
 // the single-grid sweep's register cap (8 waves per SIMD for the FP32-mixed
 // two-material tiles), which the persistent loop's few extra registers fit
-template<typename Pol, int DIR, int L, int W, bool FUSE>
+// RING: copy-kind boundaries on every instance, whose ring treatment runs
+// inside the sweep (sweep_tile)
+template<typename Pol, int DIR, int L, int W, bool FUSE, bool RING>
 __global__ void __launch_bounds__(L* W) __attribute__((amdgpu_waves_per_eu(Pol::template min_waves_per_eu<2, 2>())))
 batch_sweep_kernel(const BatchInst<Pol>* __restrict__ insts, const Work* __restrict__ work, int nwork, int schedule,
                    int cur)
@@ -61,14 +63,14 @@ batch_sweep_kernel(const BatchInst<Pol>* __restrict__ insts, const Work* __restr
     for (int w = beg; w < end; w += stride) {
         const Work it = work[w];
         const BatchInst<Pol>& I = insts[it.inst];
-        sweep_tile<Pol, 2, 2, DIR, L, W, 0, FUSE, true>(I.q[cur], I.q[cur ^ 1], I.g, I.P, I.ctl, I.cfl, nullptr,
-                                                         nullptr, it.a, it.b, 0);
+        sweep_tile<Pol, 2, 2, DIR, L, W, 0, FUSE, true, RING>(I.q[cur], I.q[cur ^ 1], I.g, I.P, I.ctl, I.cfl,
+                                                               nullptr, nullptr, it.a, it.b, 0, &I.bc);
     }
 }
 
 // schedule 2: one work entry per block (no persistent loop), entries in
 // XCD order as in the single-grid sweeps
-template<typename Pol, int DIR, int L, int W, bool FUSE>
+template<typename Pol, int DIR, int L, int W, bool FUSE, bool RING>
 __global__ void __launch_bounds__(L* W) __attribute__((amdgpu_waves_per_eu(Pol::template min_waves_per_eu<2, 2>())))
 batch_tile_kernel(const BatchInst<Pol>* __restrict__ insts, const Work* __restrict__ work, int cur)
 {
@@ -76,8 +78,8 @@ batch_tile_kernel(const BatchInst<Pol>* __restrict__ insts, const Work* __restri
     xcd_block(w, unused_y, unused_z);
     const Work it = work[w];
     const BatchInst<Pol>& I = insts[it.inst];
-    sweep_tile<Pol, 2, 2, DIR, L, W, 0, FUSE, false>(I.q[cur], I.q[cur ^ 1], I.g, I.P, I.ctl, I.cfl, nullptr, nullptr,
-                                                      it.a, it.b, 0);
+    sweep_tile<Pol, 2, 2, DIR, L, W, 0, FUSE, false, RING>(I.q[cur], I.q[cur ^ 1], I.g, I.P, I.ctl, I.cfl, nullptr,
+                                                            nullptr, it.a, it.b, 0, &I.bc);
 }
 
 template<typename Pol>
@@ -145,6 +147,7 @@ batch_begin_step_kernel(const BatchInst<Pol>* __restrict__ insts, int n, int res
     ctl->dt = dt;
     ctl->dt_max = dt_max;
     ctl->active = active;
+    ctl->cfl_parity = s ^ 1;
     ctl->dtmin[s ^ 1] = kInfBits;
 }
 

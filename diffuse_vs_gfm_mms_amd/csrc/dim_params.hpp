@@ -90,6 +90,10 @@ struct StepCtl {
     double t_end;
     long long step;
     int slot;
+    // the dt-slot parity a step's fused CFL epilogue writes (slot ^ 1 when
+    // the step began; the sweeps read this, not slot, which a fused last
+    // sweep flips while it runs)
+    int cfl_parity;
     int active;
     int use_t_end;
     int err;

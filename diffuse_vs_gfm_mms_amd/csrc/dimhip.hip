@@ -284,6 +284,7 @@ __global__ void init_ctl_kernel(StepCtl* ctl)
         ctl->t_end = 0.0;
         ctl->step = 0;
         ctl->slot = 0;
+        ctl->cfl_parity = 1;
         ctl->active = 1;
         ctl->use_t_end = 0;
         ctl->err = 0;
@@ -1266,25 +1267,27 @@ const BatchInst<Pol>* insts_as(const dimhip_batch* b)
     return reinterpret_cast<const BatchInst<Pol>*>(b->d_insts);
 }
 
-template<int DIR, int L, int W, bool FUSE>
+// RING: the instances' ring treatment runs inside the sweep (copy-kind
+// boundaries); with FUSE the sweep also closes each instance's step
+template<int DIR, int L, int W, bool FUSE, bool RING>
 void launch_bsweep(dimhip_batch* b, int k)
 {
     const dim3 blk = DIR == 0 ? dim3(L, W, 1) : dim3(W, L, 1);
     if (b->schedule == 2) {
         if (b->ideal)
-            batch_tile_kernel<Fp32Ideal, DIR, L, W, FUSE><<<b->nwork[k], blk, 0, b->stream>>>(
+            batch_tile_kernel<Fp32Ideal, DIR, L, W, FUSE, RING><<<b->nwork[k], blk, 0, b->stream>>>(
                 insts_as<Fp32Ideal>(b), b->d_work[k], b->cur);
         else
-            batch_tile_kernel<BPol, DIR, L, W, FUSE><<<b->nwork[k], blk, 0, b->stream>>>(b->d_insts, b->d_work[k],
-                                                                                          b->cur);
+            batch_tile_kernel<BPol, DIR, L, W, FUSE, RING><<<b->nwork[k], blk, 0, b->stream>>>(
+                b->d_insts, b->d_work[k], b->cur);
         return;
     }
     const int g = sweep_blocks(b, b->nwork[k]);
     if (b->ideal)
-        batch_sweep_kernel<Fp32Ideal, DIR, L, W, FUSE><<<g, blk, 0, b->stream>>>(
+        batch_sweep_kernel<Fp32Ideal, DIR, L, W, FUSE, RING><<<g, blk, 0, b->stream>>>(
             insts_as<Fp32Ideal>(b), b->d_work[k], b->nwork[k], b->schedule, b->cur);
     else
-        batch_sweep_kernel<BPol, DIR, L, W, FUSE><<<g, blk, 0, b->stream>>>(
+        batch_sweep_kernel<BPol, DIR, L, W, FUSE, RING><<<g, blk, 0, b->stream>>>(
             b->d_insts, b->d_work[k], b->nwork[k], b->schedule, b->cur);
 }
 
@@ -1302,21 +1305,27 @@ void launch_bbc(dimhip_batch* b, int end_step)
     batch_bc_kernel<Pol><<<bcb, 256, 0, b->stream>>>(insts_as<Pol>(b), b->d_work[2], b->nwork[2], b->cur, end_step);
 }
 
-void batch_sweep_x(dimhip_batch* b)
+// copy-kind boundaries on every instance (fuse): ring treatment inside the
+// sweeps, the y sweep with the CFL epilogue and the step's close
+void batch_sweep_x(dimhip_batch* b, bool fuse)
 {
-    if (b->xv == 0) launch_bsweep<0, 128, 2, false>(b, 0);
-    else launch_bsweep<0, 256, 1, false>(b, 0);
+    switch (b->xv * 2 + (fuse ? 1 : 0)) {
+        case 0: launch_bsweep<0, 128, 2, false, false>(b, 0); break;
+        case 1: launch_bsweep<0, 128, 2, false, true>(b, 0); break;
+        case 2: launch_bsweep<0, 256, 1, false, false>(b, 0); break;
+        default: launch_bsweep<0, 256, 1, false, true>(b, 0); break;
+    }
 }
 
 void batch_sweep_y(dimhip_batch* b, bool fuse)
 {
     switch (b->yv * 2 + (fuse ? 1 : 0)) {
-        case 0: launch_bsweep<1, 64, 8, false>(b, 1); break;
-        case 1: launch_bsweep<1, 64, 8, true>(b, 1); break;
-        case 2: launch_bsweep<1, 32, 16, false>(b, 1); break;
-        case 3: launch_bsweep<1, 32, 16, true>(b, 1); break;
-        case 4: launch_bsweep<1, 128, 4, false>(b, 1); break;
-        default: launch_bsweep<1, 128, 4, true>(b, 1); break;
+        case 0: launch_bsweep<1, 64, 8, false, false>(b, 1); break;
+        case 1: launch_bsweep<1, 64, 8, true, true>(b, 1); break;
+        case 2: launch_bsweep<1, 32, 16, false, false>(b, 1); break;
+        case 3: launch_bsweep<1, 32, 16, true, true>(b, 1); break;
+        case 4: launch_bsweep<1, 128, 4, false, false>(b, 1); break;
+        default: launch_bsweep<1, 128, 4, true, true>(b, 1); break;
     }
 }
 
@@ -1331,22 +1340,28 @@ void batch_enqueue_step(dimhip_batch* b, double cfl)
         bev_end(b, e, KC_CFL);
     }
     batch_begin_step_kernel<BPol><<<n, kDtSlots, 0, b->stream>>>(b->d_insts, n, 0);
+    // copy-kind boundaries: the ring treatment runs inside the sweeps and the
+    // y sweep closes each instance's step; otherwise boundary passes follow
     int e = bev_begin(b);
-    batch_sweep_x(b);
+    batch_sweep_x(b, b->fusable);
     bev_end(b, e, KC_SWEEP, KC_SWEEP_X);
     b->cur ^= 1;
-    e = bev_begin(b);
-    if (b->ideal) launch_bbc<Fp32Ideal>(b, 0);
-    else launch_bbc<BPol>(b, 0);
-    bev_end(b, e, KC_BC);
+    if (!b->fusable) {
+        e = bev_begin(b);
+        if (b->ideal) launch_bbc<Fp32Ideal>(b, 0);
+        else launch_bbc<BPol>(b, 0);
+        bev_end(b, e, KC_BC);
+    }
     e = bev_begin(b);
     batch_sweep_y(b, b->fusable);
     bev_end(b, e, KC_SWEEP, KC_SWEEP_X + 1);
     b->cur ^= 1;
-    e = bev_begin(b);
-    if (b->ideal) launch_bbc<Fp32Ideal>(b, 1);
-    else launch_bbc<BPol>(b, 1);
-    bev_end(b, e, KC_BC);
+    if (!b->fusable) {
+        e = bev_begin(b);
+        if (b->ideal) launch_bbc<Fp32Ideal>(b, 1);
+        else launch_bbc<BPol>(b, 1);
+        bev_end(b, e, KC_BC);
+    }
     b->dt_ready = b->fusable;
     b->dt_ready_cfl = cfl;
     for (auto* c : b->ctx) c->cur = b->cur;

@@ -45,6 +45,7 @@ static __global__ void __launch_bounds__(64) begin_step_kernel(StepCtl* ctl, dou
     ctl->dt = dt;
     ctl->dt_max = dt_max;
     ctl->active = active;
+    ctl->cfl_parity = s ^ 1;
     ctl->dtmin[s ^ 1] = kInfBits;
 }
 

@@ -219,6 +219,163 @@ __device__ __forceinline__ void xcd_block(int& bx, int& by, int& bz)
 }
 
 // ---------------------------------------------------------------------------
+// Ring boundary treatment fused into a sweep (boundary.hpp:165-226 for the
+// copy kinds: transmissive, reflection, reflective).  Every ring cell (index
+// 0 or N-1 on an axis longer than 1) takes the state of its source cell, the
+// ring cell's index clamped to [1, N-2] on every such axis, with the normal
+// momentum of each reflective side it lies on negated.  With extent >= 3 on
+// every axis longer than 1 the source is not a ring cell, and each ring cell
+// is reached from exactly one source by moving some of the source's axes
+// from index 1 to 0 or from N-2 to N-1.  So the thread that computes a source
+// cell also stores the ring cells it feeds (one per non-empty set of such
+// moves), and ring cells are not stored by the sweep itself: the result is
+// the separate boundary pass's, without its launch or its re-read.
+// pout: the recovered-pressure plane, whose ring entries copy the source's.
+// ---------------------------------------------------------------------------
+struct BcArgs {
+    int bc_lo[3];
+    int bc_hi[3];
+    long long face_start[7];  // prefix sums of ring cells per face (d, side) = 2d+side
+};
+
+template<int D>
+__device__ __forceinline__ bool ring_cell(const Grid& g, const int* idx)
+{
+    bool r = false;
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        if (g.N[d] > 1 && (idx[d] == 0 || idx[d] == g.N[d] - 1)) r = true;
+    return r;
+}
+
+// the output cells of a tile (positions [bx*T, bx*T+T) along the sweep axis,
+// its W lines across) lie next to the ring on some axis: only such tiles
+// have ring cells or sources of ring cells (block-uniform)
+template<int D, int DIR, int T, int W>
+__device__ __forceinline__ bool ring_tile(const Grid& g, int bx, int by, int bz)
+{
+    int lo[3], hi[3];
+    if (DIR == 0) {
+        lo[0] = bx * T; hi[0] = lo[0] + T - 1;
+        lo[1] = by * W; hi[1] = lo[1] + W - 1;
+        lo[2] = bz; hi[2] = bz;
+    } else if (DIR == 1) {
+        lo[0] = bx * W; hi[0] = lo[0] + W - 1;
+        lo[1] = by * T; hi[1] = lo[1] + T - 1;
+        lo[2] = bz; hi[2] = bz;
+    } else {
+        lo[0] = bx * W; hi[0] = lo[0] + W - 1;
+        lo[1] = bz; hi[1] = bz;
+        lo[2] = by * T; hi[2] = lo[2] + T - 1;
+    }
+    bool r = false;
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        if (g.N[d] > 1 && (lo[d] <= 1 || hi[d] >= g.N[d] - 2)) r = true;
+    return r;
+}
+
+// axes on which the cell at idx is the source of the ring cell at index 0
+// (lo) or N-1 (hi)
+template<int D>
+__device__ __forceinline__ void ring_sources(const Grid& g, const int* idx, int& lo, int& hi)
+{
+    lo = 0;
+    hi = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        if (g.N[d] > 1) {
+            if (idx[d] == 1) lo |= 1 << d;
+            if (idx[d] == g.N[d] - 2) hi |= 1 << d;
+        }
+}
+
+// The ring cells a source feeds: per axis d the source's index is kept or
+// moved to the ring (to 0 where lo has bit d, to N-1 where hi has it; both
+// only for an extent of 3), so with r_d = 1 + lo_d + hi_d choices per axis
+// the source feeds prod_d r_d - 1 ring cells (usually 1; 3 at a 2D corner, 7
+// at a 3D corner).  Number c in [1, prod r_d) names one: digit d of c in
+// radix r_d, 0 keeping the index, 1 the first move, 2 the move to N-1.
+// Returns its linear index and the axes whose momentum the copy negates.
+template<int D>
+__device__ __forceinline__ int ring_count(int lo, int hi)
+{
+    int n = 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) n *= 1 + ((lo >> d) & 1) + ((hi >> d) & 1);
+    return n;
+}
+
+template<int D>
+__device__ __forceinline__ long long ring_target(const Grid& g, const BcArgs& bc, const int* idx, int lo, int hi,
+                                                 int c, int& flip)
+{
+    long long lin = 0;
+    flip = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const int l = (lo >> d) & 1, h = (hi >> d) & 1;
+        int digit = 0;
+        if (l + h == 1) {
+            digit = c & 1;
+            c >>= 1;
+        } else if (l + h == 2) {
+            digit = c % 3;
+            c /= 3;
+        }
+        int id = idx[d];
+        if (digit != 0) {
+            const bool to_lo = digit == 1 && l;
+            id = to_lo ? 0 : g.N[d] - 1;
+            if ((to_lo ? bc.bc_lo[d] : bc.bc_hi[d]) == BC_REFLECTIVE) flip |= 1 << d;
+        }
+        lin += (long long)id * g.stride[d];
+    }
+    return lin;
+}
+
+// the ring cells a source feeds get its state
+template<typename Pol, int D, int M>
+__device__ __forceinline__ void ring_copies(void* __restrict__ q, const Grid& g, const BcArgs& bc, const int* idx,
+                                            const typename Pol::template Cell<D, M>& r)
+{
+    using Cell = typename Pol::template Cell<D, M>;
+    int lo, hi;
+    ring_sources<D>(g, idx, lo, hi);
+    const int n = ring_count<D>(lo, hi);
+    for (int c = 1; c < n; ++c) {
+        int flip;
+        const long long lin = ring_target<D>(g, bc, idx, lo, hi, c, flip);
+        Cell st = r;
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            if ((flip >> d) & 1) st.mom[d] *= -1;
+        Pol::template store<D, M>(q, g.pitch, lin, st);
+    }
+}
+
+// ... and its recovered pressure
+template<int D>
+__device__ __forceinline__ void ring_pressures(double* __restrict__ pout, const Grid& g, const BcArgs& bc,
+                                               const int* idx, double pc)
+{
+    int lo, hi;
+    ring_sources<D>(g, idx, lo, hi);
+    const int n = ring_count<D>(lo, hi);
+    for (int c = 1; c < n; ++c) {
+        int flip;
+        pout[ring_target<D>(g, bc, idx, lo, hi, c, flip)] = pc;
+    }
+}
+
+__device__ __forceinline__ void end_step_ctl(StepCtl* ctl)
+{
+    ctl->time += ctl->dt;
+    ctl->step += 1;
+    ctl->slot ^= 1;
+}
+
+// ---------------------------------------------------------------------------
 // Directional sweep: sweep_direction_dispatch + solve_line
 // (sweep_core.hpp:16-202, muscl.hpp:197-329, hllc.hpp:62-195).
 //
@@ -246,13 +403,21 @@ __device__ __forceinline__ void xcd_block(int& bx, int& by, int& bz)
 // of every interior output cell, for the next sweep.
 // PERSISTENT: the block runs many tiles (the batched launches).  Its CFL
 // epilogue then reduces per wave (cfl_block_atomic's PER_WAVE).
+// RING (MODE 0, copy-kind boundaries bc only; the batched launches): the
+// ring boundary treatment is fused (ring_copies); a fused last sweep
+// (FUSE_CFL) also closes the step (time, step count, dt-slot parity), which
+// no sweep block reads: the CFL epilogue takes its parity from
+// ctl->cfl_parity, set when the step began.
 // ---------------------------------------------------------------------------
-template<typename Pol, int D, int M, int DIR, int L, int W, int MODE, bool FUSE_CFL, bool PERSISTENT = false>
+template<typename Pol, int D, int M, int DIR, int L, int W, int MODE, bool FUSE_CFL, bool PERSISTENT = false,
+         bool RING = false>
 __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* __restrict__ qout, const Grid& g,
                                            const typename Pol::Eos& P, StepCtl* __restrict__ ctl,
                                            const CflArgs& cflargs, const double* __restrict__ pin,
-                                           double* __restrict__ pout, int bx, int by, int bz)
+                                           double* __restrict__ pout, int bx, int by, int bz,
+                                           const BcArgs* __restrict__ bc = nullptr)
 {
+    static_assert(!RING || MODE == 0, "the fused ring treatment belongs to split sweeps");
     using R = typename Pol::R;
     using St = typename Pol::template St<D, M>;
     using Pr = typename Pol::template Pr<D, M>;
@@ -349,6 +514,8 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     };
     const bool end_cell = (i == 0) || (i == n - 1);
     const bool own = in && pos >= 3 && pos < L - 3;
+    // the fused ring treatment acts in tiles next to the ring only
+    const bool edge = RING && ring_tile<D, DIR, T, W>(g, bx, by, bz);
 
     if (!ctl->active) {
         // a step past t_end (or after an error): the state passes through
@@ -364,6 +531,8 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
         }
         return;
     }
+    if (RING && FUSE_CFL && bx == 0 && by == 0 && bz == 0 && threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0)
+        end_step_ctl(ctl);
 
     // phase 1
     Cell Uc;
@@ -586,22 +755,25 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
             Pol::template accumulate<D, M>(P, acc, r, Uin);
             r = acc;
         }
-        store_cell(qout, r);
+        // with the fused ring treatment a ring cell's value comes from its
+        // source's ring_copies; only tiles next to the ring hold either
+        const bool ring_out = edge && ring_cell<D>(g, idx);
+        if (!ring_out) store_cell(qout, r);
+        if (edge && !ring_out) ring_copies<Pol, D, M>(qout, g, *bc, idx, r);
         if (FUSE_CFL) {
-            bool interior = true;
-#pragma unroll
-            for (int d = 0; d < D; ++d)
-                if (g.N[d] > 1 && (idx[d] == 0 || idx[d] == g.N[d] - 1)) interior = false;
-            if (interior) {
+            if (!ring_cell<D>(g, idx)) {
                 double pc = 0.0;
                 cand = Pol::template cfl_value<D, M>(P, r, cflargs, pc);
                 if constexpr (Pol::kStoredPressure)
-                    if (pout) pout[cell] = pc;
+                    if (pout) {
+                        pout[cell] = pc;
+                        if (edge) ring_pressures<D>(pout, g, *bc, idx, pc);
+                    }
             }
         }
     }
     if (FUSE_CFL)
-        cfl_block_atomic<Pol, S, PERSISTENT>(cand, ctl, ctl->slot ^ 1,
+        cfl_block_atomic<Pol, S, PERSISTENT>(cand, ctl, ctl->cfl_parity,
                                  (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)));
 }
 
@@ -624,12 +796,6 @@ sweep_kernel(const void* __restrict__ qin, void* __restrict__ qout, Grid g, type
 // face of its lowest boundary axis; with extent >= 3 on every axis longer
 // than 1 the source cells are interior, so ring cells are independent.
 // ---------------------------------------------------------------------------
-struct BcArgs {
-    int bc_lo[3];
-    int bc_hi[3];
-    long long face_start[7];  // prefix sums of ring cells per face (d, side) = 2d+side
-};
-
 template<typename Pol, int D, int M>
 __device__ __forceinline__ void nonreflective_state(const typename Pol::Eos& P, const void* q, const Grid& g,
                                                     const int* idx, const int* src,
@@ -764,13 +930,6 @@ __device__ __forceinline__ void bc_cell(void* __restrict__ q, const Grid& g, con
         }
         Pol::template store<D, M>(q, g.pitch, lin, st);
     }
-}
-
-__device__ __forceinline__ void end_step_ctl(StepCtl* ctl)
-{
-    ctl->time += ctl->dt;
-    ctl->step += 1;
-    ctl->slot ^= 1;
 }
 
 template<typename Pol, int D, int M>

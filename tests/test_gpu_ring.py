@@ -1,0 +1,74 @@
+"""The batched form's ring boundary treatment fused into its sweeps
+(copy-kind boundaries: transmissive, reflection, reflective;
+boundary.hpp:165-226): every ring cell receives its source's new state from
+the thread that computes the source, with the normal momentum of each
+reflective side negated, and the y sweep closes each instance's step.
+Instances with small extents (3 cells: one interior cell feeds both ends of
+its axis), corners where a source feeds 3 ring cells, several tiles per
+line, and tangential velocities that the flips act on, against the same
+instances advanced alone with the separate boundary pass, bit for bit."""
+import pytest
+
+from _states import bits_equal, mismatch_report
+from diffuse_vs_gfm_mms_amd import dimhip, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [
+    ((3, 7), ("reflective", "reflection"), ("transmissive", "reflective")),
+    ((9, 3), ("reflection", "reflective"), ("reflective", "reflective")),
+    ((130, 9), ("reflective", "transmissive"), ("reflection", "reflective")),
+    ((4, 70), ("transmissive", "reflective"), ("reflective", "reflection")),
+    ((61, 66), ("reflective", "reflective"), ("reflective", "reflective")),
+]
+
+
+def _case(N, lo, hi, mats):
+    """A shock-bubble with velocity along both axes, so that each reflective
+    face flips a non-zero momentum."""
+    light = mats[1] != W.AIR
+    vel = (350.0, -40.0) if not light else (120.0, -30.0)
+    pre = (1200.0, vel, 4.0e8) if not light else (2.0, vel, 2.0e5)
+    post = (1000.0, (0.0, 0.0), 1.0e5) if not light else (1.2, (0.0, 0.0), 1.0e5)
+    return W.shock_bubble(N, (0.0, 0.0), (1.0, 1.0), list(mats), 0, 1, pre=pre, post=post,
+                          bubble=(1.0 if not light else 0.2, (-vel[0], -vel[1]), 1.0e5), centre=(0.55, 0.5),
+                          radius=0.3, shock_axis=0, shock_position=0.35, thickness=0.5 / max(N),
+                          bc_lo=lo, bc_hi=hi, name="ring_case")
+
+
+def _alone(case, nsteps):
+    with dimhip.Solver.from_case(case, precision="fp32_mixed") as s:
+        t, n = s.advance(nsteps, case.cfl, 1e300, 0.0, use_graph=False)
+        return s.download(), t, n
+
+
+@pytest.mark.parametrize("mats", ["water_air", "air_helium"])
+@pytest.mark.parametrize("cfg", [(0, 0, 2, 1), (1, 1, 0, 4), (0, 2, 1, 2)])
+def test_fused_ring_batch_equals_separate_passes(mats, cfg):
+    pair = {"water_air": (W.WATER, W.AIR), "air_helium": (W.AIR, W.HELIUM)}[mats]
+    cases = [_case(N, lo, hi, pair) for N, lo, hi in SHAPES]
+    refs = [_alone(c, 5) for c in cases]
+    with dimhip.Batch(cases) as b:
+        b.configure(*cfg)
+        times, steps = b.advance(5, 0.8, use_graph=False)
+        for i, c in enumerate(cases):
+            out = b.download(i)
+            assert steps[i] == 5, (mats, cfg, i)
+            assert times[i] == refs[i][1], (mats, cfg, i)
+            assert bits_equal(out, refs[i][0]), (mats, cfg, c.N, mismatch_report(out, refs[i][0]))
+
+
+def test_batch_with_a_nonreflective_instance_keeps_boundary_passes():
+    """One nonreflective instance turns the fused treatment off for the whole
+    batch (its ring needs two interior cells after the sweep)."""
+    pair = (W.AIR, W.HELIUM)
+    cases = [_case(N, lo, hi, pair) for N, lo, hi in SHAPES[2:]]
+    cases.append(_case((40, 24), ("nonreflective", "reflective"), ("transmissive", "nonreflective"), pair))
+    refs = [_alone(c, 5) for c in cases]
+    with dimhip.Batch(cases) as b:
+        b.configure(0, 0, 2, 1)
+        times, steps = b.advance(5, 0.8, use_graph=False)
+        for i in range(len(cases)):
+            out = b.download(i)
+            assert times[i] == refs[i][1], i
+            assert bits_equal(out, refs[i][0]), (i, mismatch_report(out, refs[i][0]))
