@@ -51,6 +51,16 @@ template<typename Pol, int D, int M>
 void PolOps<Pol, D, M>::bc(dimhip_ctx* c, void* q, int end_step, double* pstore, const Eos& P)
 {
     const long long total = c->bc.face_start[2 * D];
+    bool copy_kind = true;
+    for (int d = 0; d < D; ++d)
+        if (c->bc.bc_lo[d] == BC_NONREFLECTIVE || c->bc.bc_hi[d] == BC_NONREFLECTIVE) copy_kind = false;
+    if (copy_kind) {
+        // one value per thread: x blocks over the ring cells, y over the fields
+        const int bx = grid_blocks(total, 256);
+        const dim3 grd(bx < 2048 ? bx : 2048, 2 * M + D + (pstore ? 1 : 0), 1);
+        bc_copy_kernel<Pol, D, M><<<grd, 256, 0, c->stream>>>(q, c->grid, c->bc, c->ctl, end_step, pstore);
+        return;
+    }
     const int blocks = grid_blocks(total, 256);
     bc_kernel<Pol, D, M><<<blocks, 256, 0, c->stream>>>(q, c->grid, P, c->bc, c->ctl, end_step, pstore);
 }

@@ -1009,6 +1009,23 @@ struct Fp32MixedT {
         for (int k = 0; k < M - 1; ++k) qf[(f++) * pitch + c] = U.a[k];
     }
 
+    // stored field f (the per-cell order m[M], mom[D], E, a[M-1]) of cell src
+    // copied to cell dst, negated when neg (a bit-exact sign flip: x * -1)
+    template<int D, int M>
+    static __device__ __forceinline__ void copy_field(void* base, long long pitch, int f, long long src,
+                                                      long long dst, bool neg)
+    {
+        double* qm = (double*)base;
+        if (f < M) {
+            double* p = qm + (long long)f * pitch;
+            p[dst] = p[src];
+            return;
+        }
+        float* p = (float*)(qm + M * pitch) + (long long)(f - M) * pitch;
+        const float v = p[src];
+        p[dst] = neg ? v * -1 : v;
+    }
+
     // packed FP64 cell -> storage: volume fractions clamped to [0,1] and
     // encoded, energy shifted by B(alpha) in FP64
     template<int D, int M>

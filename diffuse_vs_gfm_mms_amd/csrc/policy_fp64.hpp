@@ -124,6 +124,17 @@ struct Fp64T {
         for (int k = 0; k < M - 1; ++k) q[(f++) * pitch + c] = U.a[k];
     }
 
+    // stored field f (the per-cell order m[M], mom[D], E, a[M-1]) of cell src
+    // copied to cell dst, negated when neg (a bit-exact sign flip: x * -1)
+    template<int D, int M>
+    static __device__ __forceinline__ void copy_field(void* base, long long pitch, int f, long long src,
+                                                      long long dst, bool neg)
+    {
+        double* p = (double*)base + (long long)f * pitch;
+        const double v = p[src];
+        p[dst] = neg ? v * -1 : v;
+    }
+
     // packed per-cell doubles <-> Cell
     template<int D, int M>
     static __device__ __forceinline__ void from_packed(const Eos&, const double* p, Cell<D, M>& U)

@@ -860,6 +860,62 @@ __device__ __forceinline__ void nonreflective_state(const typename Pol::Eos& P, 
     out = st;
 }
 
+// Ring cell number t (0 <= t < bc.face_start[2D], below 2^32): faces in the
+// order (axis, side) = 2 axis + side, each face's cells with the earlier
+// axes' ring cells left out (a cell belongs to the face of its lowest
+// boundary axis), the remaining axes ascending with the first fastest.  Its
+// index, its source (the index clamped to [1, N-2] on every axis longer than
+// 1) and both linear indices.
+template<int D>
+__device__ __forceinline__ void ring_index(const Grid& g, const BcArgs& bc, long long t, int* idx, int* src,
+                                           long long& lin, long long& sl)
+{
+    // every array index below is a compile-time constant after unrolling,
+    // so nothing is staged through private (scratch) memory
+    int face = 0;
+    long long base = 0;
+#pragma unroll
+    for (int f = 1; f < 2 * D; ++f)
+        if (t >= bc.face_start[f]) {
+            face = f;
+            base = bc.face_start[f];
+        }
+    unsigned local = (unsigned)(t - base);
+    const int fd = face >> 1, side = face & 1;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) idx[d] = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        if (d == fd) {
+            idx[d] = side ? g.N[d] - 1 : 0;
+            continue;
+        }
+        unsigned lo = 0, cnt = (unsigned)g.N[d];
+        if (d < fd && g.N[d] > 1) {
+            lo = 1;
+            cnt = (unsigned)g.N[d] - 2;
+        }
+        idx[d] = (int)(lo + local % cnt);
+        local /= cnt;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) src[d] = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        src[d] = idx[d];
+        if (g.N[d] <= 1) continue;
+        if (idx[d] == 0) src[d] = 1;
+        else if (idx[d] == g.N[d] - 1) src[d] = g.N[d] - 2;
+    }
+    lin = 0;
+    sl = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        lin += (long long)idx[d] * g.stride[d];
+        sl += (long long)src[d] * g.stride[d];
+    }
+}
+
 // boundary treatment of ring cell number t (0 <= t < bc.face_start[2D])
 template<typename Pol, int D, int M>
 __device__ __forceinline__ void bc_cell(void* __restrict__ q, const Grid& g, const typename Pol::Eos& P,
@@ -867,53 +923,15 @@ __device__ __forceinline__ void bc_cell(void* __restrict__ q, const Grid& g, con
 {
     using Cell = typename Pol::template Cell<D, M>;
     {
-        // every array index below is a compile-time constant after unrolling,
-        // so nothing is staged through private (scratch) memory
-        int face = 0;
-        long long base = 0;
-#pragma unroll
-        for (int f = 1; f < 2 * D; ++f)
-            if (t >= bc.face_start[f]) {
-                face = f;
-                base = bc.face_start[f];
-            }
-        long long local = t - base;
-        const int fd = face >> 1, side = face & 1;
-        int idx[3] = {0, 0, 0};
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            if (d == fd) {
-                idx[d] = side ? g.N[d] - 1 : 0;
-                continue;
-            }
-            int lo = 0, cnt = g.N[d];
-            if (d < fd && g.N[d] > 1) {
-                lo = 1;
-                cnt = g.N[d] - 2;
-            }
-            idx[d] = lo + (int)(local % cnt);
-            local /= cnt;
-        }
-        int src[3] = {0, 0, 0};
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            src[d] = idx[d];
-            if (g.N[d] <= 1) continue;
-            if (idx[d] == 0) src[d] = 1;
-            else if (idx[d] == g.N[d] - 1) src[d] = g.N[d] - 2;
-        }
+        int idx[3], src[3];
+        long long lin, sl;
+        ring_index<D>(g, bc, t, idx, src, lin, sl);
         // the first axis on which the cell touches the boundary names it
         int name = BC_TRANSMISSIVE;
 #pragma unroll
         for (int d = D - 1; d >= 0; --d) {
             if (idx[d] == 0) name = bc.bc_lo[d];
             else if (idx[d] == g.N[d] - 1) name = bc.bc_hi[d];
-        }
-        long long lin = 0, sl = 0;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            lin += (long long)idx[d] * g.stride[d];
-            sl += (long long)src[d] * g.stride[d];
         }
         Cell st;
         if (name == BC_NONREFLECTIVE) {
@@ -943,6 +961,43 @@ bc_kernel(void* __restrict__ q, Grid g, typename Pol::Eos P, BcArgs bc, StepCtl*
          t += (long long)gridDim.x * blockDim.x)
         bc_cell<Pol, D, M>(q, g, P, bc, t, pstore);
     if (end_step && blockIdx.x == 0 && threadIdx.x == 0 && active) end_step_ctl(ctl);
+}
+
+// The ring pass for copy-kind boundaries (no nonreflective side): the
+// copies of bc_cell, field by field.  blockIdx.y is the stored field (the
+// per-cell order m[M], mom[D], E, a[M-1]; with pstore one more for the
+// recovered pressure) and the x blocks walk the ring cells, consecutive
+// threads taking consecutive ring cells.  Each thread moves one value, so a
+// wave has 64 independent loads in flight where bc_cell's thread chains a
+// cell's fields.  Momentum d is negated on each reflective side of axis d
+// the ring cell lies on, as in bc_cell.
+template<typename Pol, int D, int M>
+__global__ void __launch_bounds__(256)
+bc_copy_kernel(void* __restrict__ q, Grid g, BcArgs bc, StepCtl* __restrict__ ctl, int end_step,
+               double* __restrict__ pstore)
+{
+    constexpr int NF = 2 * M + D;
+    const long long total = bc.face_start[2 * D];
+    const bool active = ctl->active;
+    const int f = blockIdx.y;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; active && t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        int idx[3], src[3];
+        long long lin, sl;
+        ring_index<D>(g, bc, t, idx, src, lin, sl);
+        if (f == NF) {
+            pstore[lin] = pstore[sl];
+            continue;
+        }
+        bool neg = false;
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            if (f == M + d && ((idx[d] == 0 && bc.bc_lo[d] == BC_REFLECTIVE) ||
+                               (idx[d] == g.N[d] - 1 && bc.bc_hi[d] == BC_REFLECTIVE)))
+                neg = true;
+        Pol::template copy_field<D, M>(q, g.pitch, f, sl, lin, neg);
+    }
+    if (end_step && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && active) end_step_ctl(ctl);
 }
 
 // compute_dt_cfl over every cell into ctl->dtmin[ctl->slot]

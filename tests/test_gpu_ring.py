@@ -6,10 +6,14 @@ reflective side negated, and the y sweep closes each instance's step.
 Instances with small extents (3 cells: one interior cell feeds both ends of
 its axis), corners where a source feeds 3 ring cells, several tiles per
 line, and tangential velocities that the flips act on, against the same
-instances advanced alone with the separate boundary pass, bit for bit."""
+instances advanced alone with the separate boundary pass, bit for bit.
+The single grid's separate ring pass for copy-kind boundaries (one stored
+field per thread) against the oracle, bit for bit."""
+import numpy as np
 import pytest
 
 from _states import bits_equal, mismatch_report
+from oracle.pyoracle import make_params
 from diffuse_vs_gfm_mms_amd import dimhip, workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -72,3 +76,40 @@ def test_batch_with_a_nonreflective_instance_keeps_boundary_passes():
             out = b.download(i)
             assert times[i] == refs[i][1], i
             assert bits_equal(out, refs[i][0]), (i, mismatch_report(out, refs[i][0]))
+
+
+# single-grid FP64 steps with copy-kind boundaries (the field-parallel ring
+# pass, with the recovered-pressure copies), extents down to 3, 1D to 3D
+GRID_CASES = {
+    "1d_3": ((3,), ("reflective",), ("reflection",)),
+    "1d_4": ((4,), ("transmissive",), ("reflective",)),
+    "2d_3x7": ((3, 7), ("reflective", "reflection"), ("transmissive", "reflective")),
+    "2d_130x9": ((130, 9), ("reflective", "transmissive"), ("reflection", "reflective")),
+    "3d_3x5x4": ((3, 5, 4), ("reflective", "reflection", "transmissive"),
+                 ("transmissive", "reflective", "reflective")),
+    "3d_7x3x6": ((7, 3, 6), ("reflection", "reflective", "reflective"), ("reflective", "transmissive", "reflection")),
+}
+
+
+def _grid_case(N, lo, hi):
+    dim = len(N)
+    vel = (350.0, -40.0, 25.0)[:dim]
+    return W.shock_bubble(N, (0.0,) * dim, (1.0,) * dim, [W.WATER, W.AIR], 0, 1,
+                          pre=(1200.0, vel, 4.0e8), post=(1000.0, (0.0,) * dim, 1.0e5),
+                          bubble=(1.0, tuple(-v for v in vel), 1.0e5), centre=(0.55,) * dim, radius=0.3,
+                          shock_axis=0, shock_position=0.35, thickness=0.5 / max(N), bc_lo=lo, bc_hi=hi,
+                          name="ring_grid_case")
+
+
+@pytest.mark.parametrize("name", sorted(GRID_CASES))
+def test_copy_ring_pass_fp64_matches_oracle(oracle, name):
+    N, lo, hi = GRID_CASES[name]
+    c = _grid_case(N, lo, hi)
+    p = make_params(c.dim, c.materials)
+    ref, ref_dt, _ = oracle.advance(p, c.N, c.dx, lo, hi, c.U, c.cfl, 1.0, nsteps=4)
+    with dimhip.Solver(c.dim, c.N, c.dx, c.materials, lo, hi) as s:
+        s.upload(c.U)
+        dts = [s.step(c.cfl, 1.0) for _ in range(4)]
+        out = s.download()
+    assert bits_equal(out, ref), (name, mismatch_report(out, ref))
+    assert bits_equal(np.array(dts), ref_dt), name
