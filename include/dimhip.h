@@ -260,9 +260,12 @@ const char* dimhip_last_error(const dimhip_ctx* ctx);
 
 /* ---- batched form ---------------------------------------------------------
  * Many independent 2D two-material FP32-mixed instances of uneven size
- * advanced together: each stage (CFL, x sweep, ring BCs, y sweep, ring BCs)
- * is one launch over a work table of (instance, tile) entries walked by
- * persistent blocks; every instance keeps its own dt and time on the device
+ * advanced together: each stage (x sweep, y sweep with the CFL reduction;
+ * ring boundary passes after each sweep unless every instance has
+ * copy-kind boundaries, whose ring treatment then runs inside the sweeps)
+ * is one launch over a work table of (instance, tile) entries, one block
+ * per entry or walked by persistent blocks; every instance keeps its own dt
+ * and time on the device
  * (the reference runs such sweeps as separate processes, one config each,
  * tools/quant/runner.hpp:246-371).  Results equal running each instance
  * alone through dimhip_advance. */
