@@ -409,9 +409,12 @@ __device__ __forceinline__ double internal_energy_from_pressure(const EP& P, dou
     return mixture_internal_energy_density<M>(P, p, alpha, rho) / rho_total;
 }
 
-// eos.hpp:195-221 (Wood) and eos.hpp:223-275 (allaire / equal_velocity)
+// eos.hpp:195-221 (Wood) and eos.hpp:223-275 (allaire / equal_velocity).
+// Always inlined: left to the compiler, the three-material and noble-Abel
+// sweeps called it, and the call saved their live registers to scratch
+// (368-384 B per lane) at every sound speed.
 template<int M, typename EP>
-__device__ double mixture_sound_speed(const EP& P, double rho_total, double p,
+__device__ __forceinline__ double mixture_sound_speed(const EP& P, double rho_total, double p,
                                       const double* alpha, const double* rho)
 {
     if (P.lambda_model == LAMBDA_KAPILA) {

@@ -28,10 +28,10 @@ __device__ __forceinline__ unsigned long long double_to_bits(double x) { return 
 template<bool NA>
 struct Fp64T {
     // register cap of the sweep kernels: 4 waves per SIMD (128 VGPRs); the
-    // three-material kernels (without noble-Abel) run 6 % faster at 2 waves
-    // (256 VGPRs) than with the 700 B per lane of spills the 4-wave cap costs
+    // three-material kernels (without noble-Abel) take 3 (168 VGPRs: the 2D
+    // ones fit without spills, the 4-wave cap spilled about 700 B per lane)
     template<int D, int M>
-    static constexpr int min_waves_per_eu() { return (M >= 3 && !NA) ? 2 : 4; }
+    static constexpr int min_waves_per_eu() { return (M >= 3 && !NA) ? 3 : 4; }
     // the sweep re-reads a cell's input state instead of keeping it in
     // registers across its phases (register pressure, not bandwidth, binds here)
     static constexpr bool kReloadInput = true;
