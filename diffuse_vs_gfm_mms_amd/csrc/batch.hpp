@@ -19,12 +19,13 @@ struct BatchInst {
 };
 
 // work entry: x = instance, y/z = tile coordinates (sweeps) or ring /
-// cell range [y, z) (boundary and CFL passes)
+// cell range [y, z) (boundary and CFL passes); edge: a sweep tile next to
+// the instance's ring (outputs at index 0, 1, N-2 or N-1 on some axis)
 struct Work {
     int inst;
     int a;
     int b;
-    int pad;
+    int edge;
 };
 
 // schedule: 0 = cyclic (block k takes entries k, k+G, ...), 1 = contiguous
@@ -64,7 +65,7 @@ batch_sweep_kernel(const BatchInst<Pol>* __restrict__ insts, const Work* __restr
         const Work it = work[w];
         const BatchInst<Pol>& I = insts[it.inst];
         sweep_tile<Pol, 2, 2, DIR, L, W, 0, FUSE, true, RING>(I.q[cur], I.q[cur ^ 1], I.g, I.P, I.ctl, I.cfl,
-                                                               nullptr, nullptr, it.a, it.b, 0, &I.bc);
+                                                               nullptr, nullptr, it.a, it.b, 0, &I.bc, it.edge);
     }
 }
 
@@ -79,7 +80,7 @@ batch_tile_kernel(const BatchInst<Pol>* __restrict__ insts, const Work* __restri
     const Work it = work[w];
     const BatchInst<Pol>& I = insts[it.inst];
     sweep_tile<Pol, 2, 2, DIR, L, W, 0, FUSE, false, RING>(I.q[cur], I.q[cur ^ 1], I.g, I.P, I.ctl, I.cfl, nullptr,
-                                                            nullptr, it.a, it.b, 0, &I.bc);
+                                                            nullptr, it.a, it.b, 0, &I.bc, it.edge);
 }
 
 template<typename Pol>

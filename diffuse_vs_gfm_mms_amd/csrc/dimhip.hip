@@ -1219,10 +1219,18 @@ int batch_build_work(dimhip_batch* b)
     const int Wy = kBatchYW[b->yv], Ly = kBatchYL[b->yv], Ty = Ly - 6;
     for (int i = 0; i < (int)b->ctx.size(); ++i) {
         const Grid& g = b->ctx[i]->grid;
+        // a tile whose outputs [lo0, hi0] x [lo1, hi1] reach index 0, 1, N-2
+        // or N-1 on some axis: the only tiles that hold ring cells or their
+        // sources, where the sweeps' fused ring treatment acts
+        const auto edge = [&](int lo0, int hi0, int lo1, int hi1) {
+            return (lo0 <= 1 || hi0 >= g.N[0] - 2 || lo1 <= 1 || hi1 >= g.N[1] - 2) ? 1 : 0;
+        };
         for (int by = 0; by < (g.N[1] + Wx - 1) / Wx; ++by)
-            for (int bx = 0; bx < (g.N[0] + Tx - 1) / Tx; ++bx) w[0].push_back({i, bx, by, 0});
+            for (int bx = 0; bx < (g.N[0] + Tx - 1) / Tx; ++bx)
+                w[0].push_back({i, bx, by, edge(bx * Tx, bx * Tx + Tx - 1, by * Wx, by * Wx + Wx - 1)});
         for (int by = 0; by < (g.N[1] + Ty - 1) / Ty; ++by)
-            for (int bx = 0; bx < (g.N[0] + Wy - 1) / Wy; ++bx) w[1].push_back({i, bx, by, 0});
+            for (int bx = 0; bx < (g.N[0] + Wy - 1) / Wy; ++bx)
+                w[1].push_back({i, bx, by, edge(bx * Wy, bx * Wy + Wy - 1, by * Ty, by * Ty + Ty - 1)});
         const long long ring = b->ctx[i]->bc.face_start[4];
         for (long long a = 0; a < ring; a += 1024) w[2].push_back({i, (int)a, (int)std::min<long long>(ring, a + 1024), 0});
         for (long long a = 0; a < g.ncells; a += 16384)

@@ -248,33 +248,6 @@ __device__ __forceinline__ bool ring_cell(const Grid& g, const int* idx)
     return r;
 }
 
-// the output cells of a tile (positions [bx*T, bx*T+T) along the sweep axis,
-// its W lines across) lie next to the ring on some axis: only such tiles
-// have ring cells or sources of ring cells (block-uniform)
-template<int D, int DIR, int T, int W>
-__device__ __forceinline__ bool ring_tile(const Grid& g, int bx, int by, int bz)
-{
-    int lo[3], hi[3];
-    if (DIR == 0) {
-        lo[0] = bx * T; hi[0] = lo[0] + T - 1;
-        lo[1] = by * W; hi[1] = lo[1] + W - 1;
-        lo[2] = bz; hi[2] = bz;
-    } else if (DIR == 1) {
-        lo[0] = bx * W; hi[0] = lo[0] + W - 1;
-        lo[1] = by * T; hi[1] = lo[1] + T - 1;
-        lo[2] = bz; hi[2] = bz;
-    } else {
-        lo[0] = bx * W; hi[0] = lo[0] + W - 1;
-        lo[1] = bz; hi[1] = bz;
-        lo[2] = by * T; hi[2] = lo[2] + T - 1;
-    }
-    bool r = false;
-#pragma unroll
-    for (int d = 0; d < D; ++d)
-        if (g.N[d] > 1 && (lo[d] <= 1 || hi[d] >= g.N[d] - 2)) r = true;
-    return r;
-}
-
 // axes on which the cell at idx is the source of the ring cell at index 0
 // (lo) or N-1 (hi)
 template<int D>
@@ -404,7 +377,8 @@ __device__ __forceinline__ void end_step_ctl(StepCtl* ctl)
 // PERSISTENT: the block runs many tiles (the batched launches).  Its CFL
 // epilogue then reduces per wave (cfl_block_atomic's PER_WAVE).
 // RING (MODE 0, copy-kind boundaries bc only; the batched launches): the
-// ring boundary treatment is fused (ring_copies); a fused last sweep
+// ring boundary treatment is fused (ring_copies) in the tiles the caller
+// marks with edge_tile; a fused last sweep
 // (FUSE_CFL) also closes the step (time, step count, dt-slot parity), which
 // no sweep block reads: the CFL epilogue takes its parity from
 // ctl->cfl_parity, set when the step began.
@@ -415,7 +389,7 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
                                            const typename Pol::Eos& P, StepCtl* __restrict__ ctl,
                                            const CflArgs& cflargs, const double* __restrict__ pin,
                                            double* __restrict__ pout, int bx, int by, int bz,
-                                           const BcArgs* __restrict__ bc = nullptr)
+                                           const BcArgs* __restrict__ bc = nullptr, int edge_tile = 0)
 {
     static_assert(!RING || MODE == 0, "the fused ring treatment belongs to split sweeps");
     using R = typename Pol::R;
@@ -514,8 +488,9 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     };
     const bool end_cell = (i == 0) || (i == n - 1);
     const bool own = in && pos >= 3 && pos < L - 3;
-    // the fused ring treatment acts in tiles next to the ring only
-    const bool edge = RING && ring_tile<D, DIR, T, W>(g, bx, by, bz);
+    // the fused ring treatment acts only in tiles whose outputs reach index
+    // 0, 1, N-2 or N-1 on some axis (edge_tile, from the caller's work table)
+    const bool edge = RING && edge_tile != 0;
 
     if (!ctl->active) {
         // a step past t_end (or after an error): the state passes through
