@@ -69,6 +69,11 @@ struct dimhip_ctx {
     double* flux_integral = nullptr;
     bool track_flux = false;
     bool dt_ready = false;       // ctl->dtmin[slot] holds the CFL minimum of the current state
+    // set by enqueue_step for its sweeps and ring passes: the sweeps give the
+    // x-face ring cells (other indices interior) their state and the ring
+    // pass leaves them out (3D, copy-kind boundaries, split steps, and a grid
+    // whose x-face sources share a wave with their ring cells)
+    bool xring_now = false;
     double dt_ready_cfl = 0.0;   // the cfl it was computed with
     std::string err;
     const dimhip::Ops* ops = nullptr;
@@ -108,9 +113,30 @@ struct Ops {
     virtual void prim_to_packed(dimhip_ctx* c, const double* prim, double* packed, long long count) const = 0;
     virtual void output_rows(dimhip_ctx* c, const double* packed, double* rows, long long count) const = 0;
     virtual void boundary_flux(dimhip_ctx* c, const void* q, double* integral) const = 0;
+    // the sweep tiles of the context's policy keep every x-face ring cell's
+    // source in the ring cell's wave
+    virtual bool xring_fits(const dimhip_ctx* c) const = 0;
     virtual void ic_prims(dimhip_ctx* c, const IcDesc& ic, long long first, long long count, double* prim,
                           int* err) const = 0;
 };
+
+// Tiles of policy Pol hold each x-face ring cell's source (its x-neighbour)
+// as an own cell of the same wave: x tiles (positions along x, 64 to a wave)
+// unless x = N0-1 is the first output of its tile or sits at the start of a
+// wave; y/z tiles (kSweepW1 consecutive x lines per position, lanes of one
+// wave) unless x = N0-1 starts a tile.
+template<typename Pol>
+inline bool xring_fits_tiles(const Grid& g, int dim)
+{
+    const int n0 = g.N[0];
+    if (n0 < 3) return false;
+    const int T0 = Pol::kSweepL0 - 6;
+    const int gx = (n0 + T0 - 1) / T0;
+    const int pr = (n0 - 1) - (gx - 1) * T0 + 3;  // tile position of x = N0-1
+    if (pr < 4 || pr % 64 == 0) return false;
+    if (dim >= 2 && (n0 - 1) % Pol::kSweepW1 == 0) return false;
+    return true;
+}
 
 inline CflArgs make_cfl_args(const dimhip_ctx* c, double cfl, bool unsplit)
 {

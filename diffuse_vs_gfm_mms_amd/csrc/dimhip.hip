@@ -133,6 +133,12 @@ struct OpsImpl final : Ops {
         else PolOps<Fp64, D, M>::sweep(c, dir, mode, fuse, in, out, pin, pout, c->eos);
     }
 
+    bool xring_fits(const dimhip_ctx* c) const override
+    {
+        if (c->desc.precision != DIMHIP_FP64) return xring_fits_tiles<Fp32Mixed>(c->grid, D);
+        return xring_fits_tiles<Fp64>(c->grid, D);
+    }
+
     void bc(dimhip_ctx* c, void* q, int end_step, double* pstore) const override
     {
         if (c->desc.precision != DIMHIP_FP64 && c->fp32_ideal)
@@ -354,6 +360,7 @@ void enqueue_step(dimhip_ctx* c, double cfl, double dt_max_host)
     begin_step_kernel<<<1, kDtSlots, 0, c->stream>>>(c->ctl, dt_max_host);
     const bool fuse = fusable(c);
     c->dt_ready_cfl = cfl;
+    c->xring_now = fuse && D == 3 && c->ops->xring_fits(c);
     if (!unsplit) {
         for (int dir = 0; dir < D; ++dir) {
             void* in = state_ptr(c, c->cur);
@@ -381,6 +388,7 @@ void enqueue_step(dimhip_ctx* c, double cfl, double dt_max_host)
         ev_end(c, e, KC_BC);
         c->cur ^= 1;
     }
+    c->xring_now = false;
     c->dt_ready = fuse;
     if (c->track_flux) c->ops->boundary_flux(c, state_ptr(c, c->cur), c->flux_integral);
 }

@@ -30,7 +30,11 @@ template<typename Pol, int D, int M, int DIR>
 void sweep_modes(dimhip_ctx* c, int mode, bool fuse, const void* in, void* out, const double* pin, double* pout,
                  const typename Pol::Eos& P)
 {
-    const CflArgs ca = make_cfl_args(c, c->dt_ready_cfl, c->desc.time_update == DIMHIP_UNSPLIT);
+    CflArgs ca = make_cfl_args(c, c->dt_ready_cfl, c->desc.time_update == DIMHIP_UNSPLIT);
+    if (mode == 0 && c->xring_now) {
+        ca.xring = 1;
+        ca.xflip = (c->bc.bc_lo[0] == BC_REFLECTIVE ? 1 : 0) | (c->bc.bc_hi[0] == BC_REFLECTIVE ? 2 : 0);
+    }
     if (mode == 0 && !fuse) launch_sweep<Pol, D, M, DIR, 0, false>(c, in, out, pin, nullptr, P, ca);
     else if (mode == 0 && fuse) launch_sweep<Pol, D, M, DIR, 0, true>(c, in, out, pin, pout, P, ca);
     else launch_sweep<Pol, D, M, DIR, 1, false>(c, in, out, pin, nullptr, P, ca);
@@ -58,7 +62,8 @@ void PolOps<Pol, D, M>::bc(dimhip_ctx* c, void* q, int end_step, double* pstore,
         // one value per thread: x blocks over the ring cells, y over the fields
         const int bx = grid_blocks(total, 256);
         const dim3 grd(bx < 2048 ? bx : 2048, 2 * M + D + (pstore ? 1 : 0), 1);
-        bc_copy_kernel<Pol, D, M><<<grd, 256, 0, c->stream>>>(q, c->grid, c->bc, c->ctl, end_step, pstore);
+        bc_copy_kernel<Pol, D, M><<<grd, 256, 0, c->stream>>>(q, c->grid, c->bc, c->ctl, end_step, pstore,
+                                                              c->xring_now ? 1 : 0);
         return;
     }
     const int blocks = grid_blocks(total, 256);

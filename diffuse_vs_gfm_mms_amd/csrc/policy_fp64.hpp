@@ -17,6 +17,12 @@ struct CflArgs {
     double cfl;
     double inv_cfl;
     int unsplit;
+    // single-grid 3D split steps with copy-kind boundaries: the sweeps give the
+    // x-face ring cells (other indices interior) their x-neighbour's new
+    // state, with the x momentum negated on a reflective side (xflip bit 0:
+    // low side, bit 1: high side), and the ring pass leaves them out
+    int xring;
+    int xflip;
 };
 
 constexpr unsigned long long kInfBits = 0x7FF0000000000000ull;

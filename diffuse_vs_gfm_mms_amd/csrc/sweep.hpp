@@ -308,6 +308,26 @@ __device__ __forceinline__ long long ring_target(const Grid& g, const BcArgs& bc
 }
 
 // the ring cells a source feeds get its state
+// an x-face ring cell whose other indices are interior: its source is its
+// x-neighbour, which the sweep's tile holds in the next lane of the wave
+template<int D>
+__device__ __forceinline__ bool xring_cell(const Grid& g, const int* idx)
+{
+    bool r = g.N[0] > 1 && (idx[0] == 0 || idx[0] == g.N[0] - 1);
+#pragma unroll
+    for (int d = 1; d < D; ++d)
+        if (g.N[d] > 1 && (idx[d] == 0 || idx[d] == g.N[d] - 1)) r = false;
+    return r;
+}
+
+// a tile holding the x-face outputs x = 0 or x = N0-1 (block-uniform)
+template<int DIR, int T, int W>
+__device__ __forceinline__ bool xring_tile(const Grid& g, int bx)
+{
+    const int span = DIR == 0 ? T : W;
+    return bx == 0 || (bx + 1) * span >= g.N[0];
+}
+
 template<typename Pol, int D, int M>
 __device__ __forceinline__ void ring_copies(void* __restrict__ q, const Grid& g, const BcArgs& bc, const int* idx,
                                             const typename Pol::template Cell<D, M>& r)
@@ -491,6 +511,11 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     // the fused ring treatment acts only in tiles whose outputs reach index
     // 0, 1, N-2 or N-1 on some axis (edge_tile, from the caller's work table)
     const bool edge = RING && edge_tile != 0;
+    // x-face ring cells taken from the x-neighbour's lane (cflargs.xring;
+    // 3D grids, whose x faces cost the ring pass a cache line per cell and
+    // field)
+    constexpr bool XRING = D == 3 && MODE == 0 && !RING;
+    const bool xedge = XRING && cflargs.xring && xring_tile<DIR, T, W>(g, bx);
 
     if (!ctl->active) {
         // a step past t_end (or after an error): the state passes through
@@ -730,14 +755,36 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
             Pol::template accumulate<D, M>(P, acc, r, Uin);
             r = acc;
         }
+        // x-face ring cells: the source's state from the neighbouring lane
+        // (every lane of the tile's own range is active here; the sources
+        // are own cells of the same wave)
+        bool xr = false;
+        int xsrc = 0;
+        if (XRING && xedge) {
+            const int me = __lane_id();
+            xr = xring_cell<D>(g, idx);
+            xsrc = xr ? (idx[0] == 0 ? me + 1 : me - 1) : me;
+            Cell s;
+#pragma unroll
+            for (int k = 0; k < M; ++k) s.m[k] = __shfl(r.m[k], xsrc);
+#pragma unroll
+            for (int d = 0; d < D; ++d) s.mom[d] = __shfl(r.mom[d], xsrc);
+            s.E = __shfl(r.E, xsrc);
+#pragma unroll
+            for (int k = 0; k < M - 1; ++k) s.a[k] = __shfl(r.a[k], xsrc);
+            if (xr) {
+                if ((cflargs.xflip >> (idx[0] == 0 ? 0 : 1)) & 1) s.mom[0] *= -1;
+                r = s;
+            }
+        }
         // with the fused ring treatment a ring cell's value comes from its
         // source's ring_copies; only tiles next to the ring hold either
         const bool ring_out = edge && ring_cell<D>(g, idx);
         if (!ring_out) store_cell(qout, r);
         if (edge && !ring_out) ring_copies<Pol, D, M>(qout, g, *bc, idx, r);
         if (FUSE_CFL) {
+            double pc = 0.0;
             if (!ring_cell<D>(g, idx)) {
-                double pc = 0.0;
                 cand = Pol::template cfl_value<D, M>(P, r, cflargs, pc);
                 if constexpr (Pol::kStoredPressure)
                     if (pout) {
@@ -745,6 +792,12 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
                         if (edge) ring_pressures<D>(pout, g, *bc, idx, pc);
                     }
             }
+            // an x-face ring cell's recovered pressure is its source's
+            if constexpr (Pol::kStoredPressure)
+                if (XRING && xedge && pout) {
+                    const double ps = __shfl(pc, xsrc);
+                    if (xr) pout[cell] = ps;
+                }
         }
     }
     if (FUSE_CFL)
@@ -949,7 +1002,7 @@ bc_kernel(void* __restrict__ q, Grid g, typename Pol::Eos P, BcArgs bc, StepCtl*
 template<typename Pol, int D, int M>
 __global__ void __launch_bounds__(256)
 bc_copy_kernel(void* __restrict__ q, Grid g, BcArgs bc, StepCtl* __restrict__ ctl, int end_step,
-               double* __restrict__ pstore)
+               double* __restrict__ pstore, int skip_xring)
 {
     constexpr int NF = 2 * M + D;
     const long long total = bc.face_start[2 * D];
@@ -960,6 +1013,8 @@ bc_copy_kernel(void* __restrict__ q, Grid g, BcArgs bc, StepCtl* __restrict__ ct
         int idx[3], src[3];
         long long lin, sl;
         ring_index<D>(g, bc, t, idx, src, lin, sl);
+        // x-face cells with interior other indices, done by the sweeps
+        if (skip_xring && xring_cell<D>(g, idx)) continue;
         if (f == NF) {
             pstore[lin] = pstore[sl];
             continue;

@@ -8,7 +8,8 @@ its axis), corners where a source feeds 3 ring cells, several tiles per
 line, and tangential velocities that the flips act on, against the same
 instances advanced alone with the separate boundary pass, bit for bit.
 The single grid's separate ring pass for copy-kind boundaries (one stored
-field per thread) against the oracle, bit for bit."""
+field per thread), and in 3D the sweeps' x-face ring cells taken from their
+x-neighbour's lane, against the oracle, bit for bit."""
 import numpy as np
 import pytest
 
@@ -113,3 +114,26 @@ def test_copy_ring_pass_fp64_matches_oracle(oracle, name):
         out = s.download()
     assert bits_equal(out, ref), (name, mismatch_report(out, ref))
     assert bits_equal(np.array(dts), ref_dt), name
+
+
+@pytest.mark.parametrize("name", ["3d_3x5x4", "3d_7x3x6"])
+def test_xface_ring_fp32_mixed_matches_separate_passes(name):
+    """3D steps, whose sweeps give the x-face ring cells their x-neighbour's
+    state, against the same steps taken as one sweep and one full ring pass
+    at a time (dimhip_compute_dt, dimhip_sweep, dimhip_apply_bc), bit for
+    bit in the FP32-mixed form."""
+    N, lo, hi = GRID_CASES[name]
+    c = _grid_case(N, lo, hi)
+    with dimhip.Solver(c.dim, c.N, c.dx, c.materials, lo, hi, precision="fp32_mixed") as a, \
+            dimhip.Solver(c.dim, c.N, c.dx, c.materials, lo, hi, precision="fp32_mixed") as b:
+        a.upload(c.U)
+        b.upload(c.U)
+        for k in range(3):
+            dt_a = a.step(c.cfl, 1.0)
+            dt_b = b.compute_dt(c.cfl, 1.0)
+            assert dt_a == dt_b, (name, k, dt_a, dt_b)
+            for d in range(c.dim):
+                b.sweep(d, dt_b)
+                b.apply_bc()
+            ua, ub = a.download(), b.download()
+            assert bits_equal(ua, ub), (name, k, mismatch_report(ua, ub))
