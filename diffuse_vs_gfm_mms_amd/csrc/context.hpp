@@ -74,6 +74,9 @@ struct dimhip_ctx {
     // pass leaves them out (3D, copy-kind boundaries, split steps, and a grid
     // whose x-face sources share a wave with their ring cells)
     bool xring_now = false;
+    // set by enqueue_step for a 2D split step with copy-kind boundaries: the
+    // x sweep writes the y-face ring rows and no ring pass follows it
+    bool yring_now = false;
     double dt_ready_cfl = 0.0;   // the cfl it was computed with
     std::string err;
     const dimhip::Ops* ops = nullptr;

@@ -361,6 +361,7 @@ void enqueue_step(dimhip_ctx* c, double cfl, double dt_max_host)
     const bool fuse = fusable(c);
     c->dt_ready_cfl = cfl;
     c->xring_now = fuse && D == 3 && c->ops->xring_fits(c);
+    c->yring_now = fuse && D == 2 && c->grid.N[0] >= 3 && c->grid.N[1] >= 3;
     if (!unsplit) {
         for (int dir = 0; dir < D; ++dir) {
             void* in = state_ptr(c, c->cur);
@@ -369,9 +370,14 @@ void enqueue_step(dimhip_ctx* c, double cfl, double dt_max_host)
             int e = ev_begin(c);
             c->ops->sweep(c, dir, 0, fuse && last, in, out, dir == 0 ? ps : nullptr, fuse && last ? ps : nullptr);
             ev_end(c, e, KC_SWEEP, KC_SWEEP_X + dir);
-            e = ev_begin(c);
-            c->ops->bc(c, out, last, fuse && last ? ps : nullptr);
-            ev_end(c, e, KC_BC);
+            // 2D, copy-kind boundaries: the x sweep wrote the y-face ring rows
+            // the y sweep reads, and the pass after the y sweep restores the
+            // rest of the ring
+            if (!(c->yring_now && dir == 0)) {
+                e = ev_begin(c);
+                c->ops->bc(c, out, last, fuse && last ? ps : nullptr);
+                ev_end(c, e, KC_BC);
+            }
             c->cur ^= 1;
         }
     } else {
@@ -389,6 +395,7 @@ void enqueue_step(dimhip_ctx* c, double cfl, double dt_max_host)
         c->cur ^= 1;
     }
     c->xring_now = false;
+    c->yring_now = false;
     c->dt_ready = fuse;
     if (c->track_flux) c->ops->boundary_flux(c, state_ptr(c, c->cur), c->flux_integral);
 }

@@ -35,6 +35,10 @@ void sweep_modes(dimhip_ctx* c, int mode, bool fuse, const void* in, void* out, 
         ca.xring = 1;
         ca.xflip = (c->bc.bc_lo[0] == BC_REFLECTIVE ? 1 : 0) | (c->bc.bc_hi[0] == BC_REFLECTIVE ? 2 : 0);
     }
+    if (mode == 0 && c->yring_now && D == 2) {
+        ca.yring = 1;
+        ca.yflip = (c->bc.bc_lo[1] == BC_REFLECTIVE ? 1 : 0) | (c->bc.bc_hi[1] == BC_REFLECTIVE ? 2 : 0);
+    }
     if (mode == 0 && !fuse) launch_sweep<Pol, D, M, DIR, 0, false>(c, in, out, pin, nullptr, P, ca);
     else if (mode == 0 && fuse) launch_sweep<Pol, D, M, DIR, 0, true>(c, in, out, pin, pout, P, ca);
     else launch_sweep<Pol, D, M, DIR, 1, false>(c, in, out, pin, nullptr, P, ca);

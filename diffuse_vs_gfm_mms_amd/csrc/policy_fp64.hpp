@@ -23,6 +23,13 @@ struct CflArgs {
     // low side, bit 1: high side), and the ring pass leaves them out
     int xring;
     int xflip;
+    // single-grid 2D split steps with copy-kind boundaries: the x sweep
+    // writes the y-face ring rows (x interior) from rows 1 and N1-2, with
+    // the y momentum negated on a reflective side (yflip bit 0: low, bit 1:
+    // high), and no ring pass follows it (the y sweep reads no other ring
+    // cell for an interior output; the pass after it restores the whole ring)
+    int yring;
+    int yflip;
 };
 
 constexpr unsigned long long kInfBits = 0x7FF0000000000000ull;

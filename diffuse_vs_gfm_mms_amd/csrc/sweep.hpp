@@ -516,6 +516,10 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
     // field)
     constexpr bool XRING = D == 3 && MODE == 0 && !RING;
     const bool xedge = XRING && cflargs.xring && xring_tile<DIR, T, W>(g, bx);
+    // y-face ring rows written by the 2D x sweep (cflargs.yring), in the
+    // tiles whose lines include y = 0, 1, N1-2 or N1-1
+    constexpr bool YRING = D == 2 && DIR == 0 && MODE == 0 && !RING;
+    const bool yedge = YRING && cflargs.yring && (by * W <= 1 || (by + 1) * W >= g.N[1] - 1);
 
     if (!ctl->active) {
         // a step past t_end (or after an error): the state passes through
@@ -777,10 +781,29 @@ __device__ __forceinline__ void sweep_tile(const void* __restrict__ qin, void* _
                 r = s;
             }
         }
+        // y-face ring rows (2D x sweep): rows 1 and N1-2 store their copies
+        // for interior x, and the ring rows store nothing themselves
+        bool yskip = false;
+        if (YRING && yedge) {
+            const int y = idx[1];
+            yskip = y == 0 || y == g.N[1] - 1;
+            if (idx[0] >= 1 && idx[0] <= g.N[0] - 2) {
+                if (y == 1) {
+                    Cell s = r;
+                    if (cflargs.yflip & 1) s.mom[1] *= -1;
+                    Pol::template store<D, M>(qout, g.pitch, cell - g.stride[1], s);
+                }
+                if (y == g.N[1] - 2) {
+                    Cell s = r;
+                    if (cflargs.yflip & 2) s.mom[1] *= -1;
+                    Pol::template store<D, M>(qout, g.pitch, cell + g.stride[1], s);
+                }
+            }
+        }
         // with the fused ring treatment a ring cell's value comes from its
         // source's ring_copies; only tiles next to the ring hold either
         const bool ring_out = edge && ring_cell<D>(g, idx);
-        if (!ring_out) store_cell(qout, r);
+        if (!ring_out && !yskip) store_cell(qout, r);
         if (edge && !ring_out) ring_copies<Pol, D, M>(qout, g, *bc, idx, r);
         if (FUSE_CFL) {
             double pc = 0.0;

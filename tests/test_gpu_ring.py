@@ -8,8 +8,9 @@ its axis), corners where a source feeds 3 ring cells, several tiles per
 line, and tangential velocities that the flips act on, against the same
 instances advanced alone with the separate boundary pass, bit for bit.
 The single grid's separate ring pass for copy-kind boundaries (one stored
-field per thread), and in 3D the sweeps' x-face ring cells taken from their
-x-neighbour's lane, against the oracle, bit for bit."""
+field per thread), in 3D the sweeps' x-face ring cells taken from their
+x-neighbour's lane, and in 2D the y-face ring rows written by the x sweep
+(no ring pass between the two sweeps), against the oracle, bit for bit."""
 import numpy as np
 import pytest
 
@@ -86,6 +87,8 @@ GRID_CASES = {
     "1d_4": ((4,), ("transmissive",), ("reflective",)),
     "2d_3x7": ((3, 7), ("reflective", "reflection"), ("transmissive", "reflective")),
     "2d_130x9": ((130, 9), ("reflective", "transmissive"), ("reflection", "reflective")),
+    "2d_61x66": ((61, 66), ("reflection", "reflective"), ("reflective", "transmissive")),
+    "2d_300x5": ((300, 5), ("transmissive", "reflective"), ("reflective", "reflection")),
     "3d_3x5x4": ((3, 5, 4), ("reflective", "reflection", "transmissive"),
                  ("transmissive", "reflective", "reflective")),
     "3d_7x3x6": ((7, 3, 6), ("reflection", "reflective", "reflective"), ("reflective", "transmissive", "reflection")),
@@ -116,16 +119,18 @@ def test_copy_ring_pass_fp64_matches_oracle(oracle, name):
     assert bits_equal(np.array(dts), ref_dt), name
 
 
-@pytest.mark.parametrize("name", ["3d_3x5x4", "3d_7x3x6"])
-def test_xface_ring_fp32_mixed_matches_separate_passes(name):
-    """3D steps, whose sweeps give the x-face ring cells their x-neighbour's
-    state, against the same steps taken as one sweep and one full ring pass
-    at a time (dimhip_compute_dt, dimhip_sweep, dimhip_apply_bc), bit for
-    bit in the FP32-mixed form."""
+@pytest.mark.parametrize("precision", ["fp32_mixed", "fp64"])
+@pytest.mark.parametrize("name", ["2d_3x7", "2d_130x9", "2d_61x66", "2d_300x5", "3d_3x5x4", "3d_7x3x6"])
+def test_fused_ring_steps_match_separate_passes(name, precision):
+    """Steps whose sweeps carry part of the ring treatment (3D: x-face ring
+    cells from their x-neighbour's lane; 2D: the y-face ring rows written by
+    the x sweep, with no ring pass before the y sweep) against the same steps
+    taken as one sweep and one full ring pass at a time (dimhip_compute_dt,
+    dimhip_sweep, dimhip_apply_bc), bit for bit."""
     N, lo, hi = GRID_CASES[name]
     c = _grid_case(N, lo, hi)
-    with dimhip.Solver(c.dim, c.N, c.dx, c.materials, lo, hi, precision="fp32_mixed") as a, \
-            dimhip.Solver(c.dim, c.N, c.dx, c.materials, lo, hi, precision="fp32_mixed") as b:
+    with dimhip.Solver(c.dim, c.N, c.dx, c.materials, lo, hi, precision=precision) as a, \
+            dimhip.Solver(c.dim, c.N, c.dx, c.materials, lo, hi, precision=precision) as b:
         a.upload(c.U)
         b.upload(c.U)
         for k in range(3):
