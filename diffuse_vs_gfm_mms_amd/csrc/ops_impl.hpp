@@ -22,7 +22,9 @@ void launch_sweep(dimhip_ctx* c, const void* in, void* out, const double* pin, d
         constexpr int L = Pol::kSweepL1, W = Pol::kSweepW1, T = L - 6;
         dim3 blk(W, L, 1);
         dim3 grd((g.N[0] + W - 1) / W, (g.N[DIR] + T - 1) / T, DIR == 1 ? g.N[2] : g.N[1]);
-        sweep_kernel<Pol, D, M, DIR, L, W, MODE, FUSE><<<grd, blk, 0, c->stream>>>(in, out, g, P, c->ctl, ca, pin, pout);
+        CflArgs a = ca;
+        if (Pol::kXcdStrips && D == 2) a.band = (int)((grd.x + 7) / 8);
+        sweep_kernel<Pol, D, M, DIR, L, W, MODE, FUSE><<<grd, blk, 0, c->stream>>>(in, out, g, P, c->ctl, a, pin, pout);
     }
 }
 

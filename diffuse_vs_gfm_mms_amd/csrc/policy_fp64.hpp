@@ -30,6 +30,9 @@ struct CflArgs {
     // cell for an interior output; the pass after it restores the whole ring)
     int yring;
     int yflip;
+    // y and z sweeps: block order in column strips of `band` tiles
+    // (xcd_block_banded); 0 keeps the row-major order of xcd_block
+    int band;
 };
 
 constexpr unsigned long long kInfBits = 0x7FF0000000000000ull;
@@ -63,6 +66,9 @@ struct Fp64T {
     // sweep tiles: x 128 positions x 2 lines; y/z 4 lines x 128 positions
     // (halo 6 of 128; 8 lines x 64 positions was 1.6 % slower)
     static constexpr int kSweepL0 = 128, kSweepW0 = 2, kSweepL1 = 128, kSweepW1 = 4;
+    // 2D y sweeps keep the row-major block order: one column strip per XCD
+    // (xcd_block_banded) measured 3 % slower for these VALU-bound tiles
+    static constexpr bool kXcdStrips = false;
     // CFL reduction: each cell contributes its dt candidate, combined by min
     static constexpr bool kCflRate = false;
     // the CFL pass stores each cell's recovered pressure (cons_to_prim's p,

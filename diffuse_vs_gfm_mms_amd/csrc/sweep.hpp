@@ -218,6 +218,39 @@ __device__ __forceinline__ void xcd_block(int& bx, int& by, int& bz)
     bz = (int)(l / (gx * gy));
 }
 
+// The same per-XCD logical index, walked in column strips: strip s holds
+// block columns [s*band, (s+1)*band) (the last strip what remains), each strip
+// row by row.  With band = ceil(gx/8) each XCD's share of a 2D grid is about
+// one strip, so a y tile's halo rows, which the tiles above and below it read
+// too, stay in that XCD's L2.  A bijection for any grid and band >= 1.
+__device__ __forceinline__ void xcd_block_banded(int& bx, int& by, int& bz, unsigned band)
+{
+    const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+    const unsigned total = gx * gy * gz;
+    const unsigned p = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned x = p & 7u, q = total >> 3, r = total & 7u;
+    const unsigned l = x * q + (x < r ? x : r) + (p >> 3);
+    const unsigned plane = gx * gy;
+    const unsigned z = l / plane;
+    const unsigned m = l - z * plane;
+    const unsigned full = gx / band;
+    const unsigned fc = full * band * gy;
+    unsigned ox, oy;
+    if (m < fc) {
+        const unsigned t = m / band;
+        const unsigned b = t / gy;
+        oy = t - b * gy;
+        ox = b * band + (m - t * band);
+    } else {
+        const unsigned w = gx - full * band, rr = m - fc;
+        oy = rr / w;
+        ox = full * band + (rr - oy * w);
+    }
+    bx = (int)ox;
+    by = (int)oy;
+    bz = (int)z;
+}
+
 // ---------------------------------------------------------------------------
 // Ring boundary treatment fused into a sweep (boundary.hpp:165-226 for the
 // copy kinds: transmissive, reflection, reflective).  Every ring cell (index
@@ -838,7 +871,8 @@ sweep_kernel(const void* __restrict__ qin, void* __restrict__ qout, Grid g, type
              StepCtl* __restrict__ ctl, CflArgs cflargs, const double* __restrict__ pin, double* __restrict__ pout)
 {
     int bx, by, bz;
-    xcd_block(bx, by, bz);
+    if (DIR >= 1 && cflargs.band > 0) xcd_block_banded(bx, by, bz, (unsigned)cflargs.band);
+    else xcd_block(bx, by, bz);
     sweep_tile<Pol, D, M, DIR, L, W, MODE, FUSE_CFL>(qin, qout, g, P, ctl, cflargs, pin, pout, bx, by, bz);
 }
 
